@@ -1,0 +1,198 @@
+/*
+ * ludwig_hip.h -- C ABI of the MI355X (gfx950) lattice-Boltzmann time step
+ * (libludwig_hip.so).
+ *
+ * Drop-in seam: the Julia host of OpenLUDWIG would `ccall` these in place of
+ *   - execute_timestep_batch!  (src/solver/solver_control.jl:199-219), which
+ *     drives recursive_step! (:6-197) -> perform_timestep_v2!
+ *     (src/collision/timestep.jl:1-159) -> stream_collide_cumulant!
+ *     (src/collision/kernel_cumulant.jl:29-627), apply_bouzidi_correction!
+ *     (src/geometry/bouzidi_kernel.jl:99-123) and stream_collide_thermal!
+ *     (src/collision/kernel_thermal.jl:14-202);
+ *   - compute_aerodynamics!    (src/forces/surface.jl:913-919);
+ *   - adapt(backend, BlockLevel) (src/main.jl:103), init_eq! and
+ *     init_thermal_eq! (src/main.jl:113-172);
+ *   - the downloads of io_vtk.jl:46-51 and diagnostics.jl:43-93.
+ *
+ * Conventions
+ *   - Host arrays are borrowed for the duration of a call and copied; the
+ *     caller keeps ownership.  The context owns all device memory.
+ *   - Tables use the reference's own values and shapes (1-based, 0 = absent;
+ *     column-major), so Julia arrays pass unchanged.
+ *   - Every function returns 0 on success or a negative LB_E* code; the message
+ *     is in lb_last_error(ctx).  No C++ exception crosses the ABI.
+ *   - A context is externally synchronised (one host thread at a time) and
+ *     issues all device work on one HIP stream.
+ */
+#ifndef LUDWIG_HIP_H
+#define LUDWIG_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LB_OK 0
+#define LB_EINVAL (-1)
+#define LB_EHIP (-2)
+#define LB_ENOMEM (-3)
+#define LB_ESTATE (-4)
+#define LB_ENONFINITE (-5)
+
+#define LB_STORAGE_FP32 0
+#define LB_STORAGE_FP16 1 /* populations stored as f - w in binary16 */
+#define LB_STORAGE_BF16 2 /* populations stored as f - w in bfloat16 */
+
+typedef struct lb_ctx lb_ctx;
+
+/* Case-wide configuration fixed at creation (config_loader.jl globals that the
+ * launcher reads: TEMPORAL_INTERPOLATION, DDF_ENABLED, Q_MIN_THRESHOLD).     */
+typedef struct {
+    int32_t n_levels;
+    int32_t domain_nx, domain_ny, domain_nz; /* coarse cells (physics_scaling.jl:119-121) */
+    int32_t temporal_interpolation;          /* TEMPORAL_INTERPOLATION                   */
+    int32_t ddf_enabled;                     /* DDF_ENABLED                              */
+    float q_min_threshold;                   /* Q_MIN_THRESHOLD                          */
+    int32_t storage;                         /* LB_STORAGE_*                             */
+    int32_t strict_fp;                       /* 1: no FMA contraction (bitwise-comparable) */
+    int32_t reserved[7];
+} lb_config;
+
+/* One BlockLevel (src/geometry/blocks.jl:16-74) as the reference holds it. */
+typedef struct {
+    int32_t level_id; /* 1-based                                              */
+    int32_t n_blocks;
+    const int32_t* map_x; /* (n_b) 1-based block coords, sorted (bx,by,bz)   */
+    const int32_t* map_y;
+    const int32_t* map_z;
+    const int32_t* neighbor_table; /* (n_b,27) column-major, 1-based, 0 = none */
+    const int32_t* block_pointer;  /* (ptr_dim_x,ptr_dim_y,ptr_dim_z) col-major */
+    int32_t ptr_dim_x, ptr_dim_y, ptr_dim_z;
+    const uint8_t* obstacle; /* (8,8,8,n_b) Bool                             */
+    const float* sponge;     /* (8,8,8,n_b)                                   */
+    const float* wall_dist;  /* (8,8,8,n_b) or NULL when the wall model is off */
+    float tau;               /* level.tau                                     */
+    float tau_g;             /* level.tau_g (thermal)                          */
+    double dx;               /* level.dx (metres), used by lb_forces           */
+    /* Bouzidi (bouzidi_enabled levels): the sparse cell list of
+     * compute_bouzidi_qmap_sparse and either the dense q map
+     * (8,8,8,n_b,27) or q per listed cell (n_bc,27); FP16 bits.          */
+    int64_t n_boundary_cells;
+    const int32_t* bouzidi_cell_block; /* (n_bc) 1-based                       */
+    const int8_t* bouzidi_cell_x;      /* (n_bc) 1-based                       */
+    const int8_t* bouzidi_cell_y;
+    const int8_t* bouzidi_cell_z;
+    const uint16_t* bouzidi_q_dense;  /* may be NULL if bouzidi_q_cells is set */
+    const uint16_t* bouzidi_q_cells;  /* may be NULL if bouzidi_q_dense is set */
+} lb_level_desc;
+
+/* Per-call scalars of execute_timestep_batch! and the globals read inside
+ * perform_timestep_v2! (timestep.jl:40-86, kernel_cumulant.jl:29-69).      */
+typedef struct {
+    float c_wale;           /* c_wale (C_SMAGO)                          */
+    float nu_sgs_background;
+    float inlet_turbulence;
+    int32_t symmetric;      /* SYMMETRIC_ANALYSIS                         */
+    int32_t wall_model;     /* params.wall_model_active                   */
+    int32_t sponge_blend_distributions;
+    float omega_bulk, omega_3, omega_4;
+    int32_t adaptive_omega4;
+    float lambda_param;
+    int32_t limiter;        /* 0 none, 1 factored, 2 positivity            */
+    int32_t compressible;   /* COMPRESSIBILITY_CORRECTION                  */
+    float T_inlet, T_wall;  /* DDF_T_INLET, DDF_T_WALL                      */
+    int32_t wall_bc_adiabatic;
+    int32_t reserved[8];
+} lb_step_params;
+
+/* compute_aerodynamics! inputs (surface.jl:766-919, structs.jl:44-74).     */
+typedef struct {
+    double mesh_offset[3];
+    double velocity_scale; /* params.velocity_scale                         */
+    double rho_physical;
+    double rho_ref, u_ref, area_ref, chord_ref;
+    double moment_center[3];
+    float c_wale;            /* C_SMAGO                                      */
+    float nu_sgs_background; /* NU_SGS_BACKGROUND                            */
+    int32_t search_radius;   /* 5 at every call site (main.jl:238)           */
+    int32_t symmetric;
+    int32_t reserved[8];
+} lb_force_params;
+
+/* lb_forces output slots */
+#define LB_F_FXP 0
+#define LB_F_FYP 1
+#define LB_F_FZP 2
+#define LB_F_FXV 3
+#define LB_F_FYV 4
+#define LB_F_FZV 5
+#define LB_F_MX 6
+#define LB_F_MY 7
+#define LB_F_MZ 8
+#define LB_F_FX 9
+#define LB_F_FY 10
+#define LB_F_FZ 11
+#define LB_F_CD 12
+#define LB_F_CL 13
+#define LB_F_CS 14
+#define LB_F_CMX 15
+#define LB_F_CMY 16
+#define LB_F_CMZ 17
+#define LB_F_N 18
+
+/* lb_download_level / lb_upload_level fields (reference layout, FP32) */
+#define LB_FIELD_F 0   /* (8,8,8,n_b,27) latest populations          */
+#define LB_FIELD_RHO 1 /* (8,8,8,n_b) level.rho                        */
+#define LB_FIELD_VEL 2 /* (8,8,8,n_b,3) latest velocity                */
+#define LB_FIELD_G 3   /* (8,8,8,n_b,27) latest thermal populations    */
+#define LB_FIELD_T 4   /* (8,8,8,n_b) level.temperature                */
+
+const char* lb_last_error(const lb_ctx* ctx);
+const char* lb_version(void);
+
+/* Create a context on HIP device `device`.  Replaces the backend selection and
+ * allocation of solve_main (main.jl:49-51, 103).                          */
+int lb_create(lb_ctx** ctx, int device, const lb_config* cfg);
+void lb_destroy(lb_ctx* ctx);
+
+/* adapt(backend, BlockLevel) for one level (main.jl:103, blocks.jl:76-100).
+ * Levels must be uploaded in order 1..n_levels.                           */
+int lb_level_upload(lb_ctx* ctx, const lb_level_desc* desc);
+
+/* init_eq! + init_thermal_eq! for every level (main.jl:113-172).          */
+int lb_init_equilibrium(lb_ctx* ctx, float T0);
+
+/* execute_timestep_batch!(grids, t_start, n_steps, u_curr, ...)
+ * (solver_control.jl:199-219): n_steps coarse steps t_start..t_start+n-1,
+ * each the full 2:1 multilevel recursion.  Asynchronous on the context's
+ * stream; lb_synchronize waits.                                            */
+int lb_step_batch(lb_ctx* ctx, int64_t t_start, int32_t n_steps, float u_curr, const lb_step_params* p);
+int lb_synchronize(lb_ctx* ctx);
+
+/* upload_mesh_to_gpu (geometry.jl:50-73): FP32 centres, normals, areas.    */
+int lb_mesh_upload(lb_ctx* ctx, int64_t n_tri, const float* cx, const float* cy, const float* cz, const float* nx,
+                   const float* ny, const float* nz, const float* areas);
+
+/* compute_aerodynamics! on the finest level (surface.jl:913-919), with the
+ * integration reduced in FP64 in a fixed order (deterministic).  out has
+ * LB_F_N doubles.  Optional maps (may be NULL): per-triangle p, tau_x/y/z. */
+int lb_forces(lb_ctx* ctx, const lb_force_params* fp, double* out, float* p_map, float* sx_map, float* sy_map,
+              float* sz_map);
+
+/* Field transfer in the reference layout (io_vtk.jl:46-51).               */
+int lb_download_level(lb_ctx* ctx, int level_id, int field, float* dst);
+int lb_upload_level(lb_ctx* ctx, int level_id, int field, const float* src);
+
+/* FP64 diagnostics over fluid cells of a level (diagnostics.jl:43-93, plus
+ * the global mass/energy check): out[0] sum rho, [1] sum 0.5 rho u^2,
+ * [2] rho min, [3] rho max, [4] number of non-finite populations,
+ * [5] fluid cell count, [6] sum of all populations (mass), [7] reserved.  */
+int lb_stats(lb_ctx* ctx, int level_id, double* out);
+
+/* Device bytes the context holds (serving admission).                    */
+int64_t lb_device_bytes(const lb_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

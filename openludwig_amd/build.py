@@ -1,0 +1,100 @@
+"""Build the in-tree native libraries of openludwig_amd.
+
+libludwig_hip.so   gfx950 kernels + C ABI (include/ludwig_hip.h)
+libludwig_setup.so host table producers (include/ludwig_setup.h)
+
+The kernel source is compiled twice: with ``-ffp-contract=off`` (namespace
+lbk_strict, every float operation rounded as written) and with contraction
+(lbk_fast).  Packed FP32 instructions are disabled for all device code.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INC = ROOT / "include"
+OUT = PKG / "_build"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+DEVICE_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+    f"-I{INC}",
+    f"-I{CSRC}",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout)
+        raise RuntimeError(f"build step failed ({res.returncode}): {' '.join(cmd)}")
+    # hipcc's host pass warns about the device-only target feature; keep other output
+    for line in res.stdout.splitlines():
+        if "packed-fp32-ops" in line or "not a recognized feature" in line:
+            continue
+        if line.strip():
+            sys.stderr.write(line + "\n")
+
+
+def _newer(target: Path, deps: list[Path]) -> bool:
+    if not target.exists():
+        return False
+    t = target.stat().st_mtime
+    return all(d.stat().st_mtime <= t for d in deps)
+
+
+def build_hip(force: bool = True) -> Path:
+    OUT.mkdir(exist_ok=True)
+    lib = OUT / "libludwig_hip.so"
+    deps = [CSRC / "lb_kernels.hip", CSRC / "lb_api.hip", CSRC / "lb_kernels.h", INC / "ludwig_hip.h"]
+    if not force and _newer(lib, deps):
+        return lib
+    objs = []
+    for ns, contract in (("lbk_strict", "off"), ("lbk_fast", "fast")):
+        obj = OUT / f"lb_kernels_{ns}.o"
+        _run([HIPCC, *DEVICE_FLAGS, f"-ffp-contract={contract}", f"-DLBK_NS={ns}", "-c",
+              str(CSRC / "lb_kernels.hip"), "-o", str(obj)])
+        objs.append(obj)
+    api = OUT / "lb_api.o"
+    _run([HIPCC, *DEVICE_FLAGS, "-ffp-contract=off", "-c", str(CSRC / "lb_api.hip"), "-o", str(api)])
+    objs.append(api)
+    tmp = OUT / "libludwig_hip.so.tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    shutil.move(str(tmp), str(lib))
+    return lib
+
+
+def build_setup(force: bool = True) -> Path:
+    OUT.mkdir(exist_ok=True)
+    lib = OUT / "libludwig_setup.so"
+    deps = [CSRC / "lb_setup.cpp", INC / "ludwig_setup.h"]
+    if not force and _newer(lib, deps):
+        return lib
+    tmp = OUT / "libludwig_setup.so.tmp"
+    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
+          "-Wall", f"-I{INC}", str(CSRC / "lb_setup.cpp"), "-o", str(tmp)])
+    shutil.move(str(tmp), str(lib))
+    return lib
+
+
+def build_all(force: bool = True) -> None:
+    build_setup(force)
+    build_hip(force)
+
+
+if __name__ == "__main__":
+    build_all(force="--if-stale" not in sys.argv)
+    print("built", OUT)

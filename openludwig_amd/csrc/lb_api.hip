@@ -1,0 +1,754 @@
+// lb_api.hip -- C ABI of libludwig_hip.so (declared in include/ludwig_hip.h).
+//
+// Host side of the time step: device state per level, the 2:1 multilevel
+// recursion of execute_timestep_batch! / recursive_step!
+// (src/solver/solver_control.jl:6-219), and the layout conversions between
+// the reference's column-major arrays and the device's block-major SoA.
+//
+// Differences in data flow from the reference, each exact:
+//   * f_old / vel_old / g_old are not copies: a level with children keeps its
+//     step's input buffers (f_in, vel_in, g_in) untouched until its next step,
+//     which comes after both child substeps, so the children read them in
+//     place (copy_to_old!, src/geometry/blocks.jl:241-252).  rho and T, single
+//     buffers in the reference, are ping-ponged by step parity so the previous
+//     step's values survive as rho_old / T_old.
+//   * f_post_collision is not allocated: Bouzidi gathers its inputs from the
+//     uncorrected f_out before scattering the corrected slots.
+//   * No device synchronisation between launches; everything is ordered on
+//     the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ludwig_hip.h"
+#include "lb_kernels.h"
+
+namespace {
+
+constexpr int NC = 512;
+constexpr int Q = 27;
+
+struct Kern {
+    hipError_t (*collide)(const lbk::CollideArgs&, hipStream_t);
+    hipError_t (*thermal)(const lbk::ThermalArgs&, hipStream_t);
+    hipError_t (*bouzidi)(const lbk::BouzidiLinks&, float*, hipStream_t);
+    hipError_t (*init_level)(float*, float*, float*, float*, float*, float*, int32_t, hipStream_t);
+    hipError_t (*init_thermal)(float*, float*, float*, float*, int32_t, float, hipStream_t);
+    hipError_t (*forces)(const lbk::ForceArgs&, int, double*, hipStream_t);
+    hipError_t (*stats)(const float*, const float*, const float*, const uint8_t*, int32_t, double*, int, double*,
+                        hipStream_t);
+};
+
+const Kern kStrict = {lbk_strict::collide, lbk_strict::thermal, lbk_strict::bouzidi, lbk_strict::init_level,
+                      lbk_strict::init_thermal, lbk_strict::forces, lbk_strict::stats};
+const Kern kFast = {lbk_fast::collide, lbk_fast::thermal, lbk_fast::bouzidi, lbk_fast::init_level,
+                    lbk_fast::init_thermal, lbk_fast::forces, lbk_fast::stats};
+
+struct Level {
+    bool uploaded = false;
+    int id = 0;
+    int32_t nb = 0;
+    double dx = 0;
+    float tau = 0, tau_g = 0;
+    float* f[2] = {nullptr, nullptr};
+    float* rho[2] = {nullptr, nullptr};
+    float* vel[2] = {nullptr, nullptr};
+    float* g[2] = {nullptr, nullptr};
+    float* T[2] = {nullptr, nullptr};
+    uint8_t* obst = nullptr;
+    float* sponge = nullptr;
+    float* wall_dist = nullptr;
+    int32_t* nbr = nullptr;
+    int32_t* borg = nullptr;
+    int32_t* grid = nullptr;
+    int32_t pd[3] = {0, 0, 0};
+    // Bouzidi link list
+    int64_t nlinks = 0;
+    int64_t* ldst = nullptr;
+    int64_t* ls1 = nullptr;
+    int64_t* ls2 = nullptr;
+    float* la = nullptr;
+    float* lb = nullptr;
+    float* ltmp = nullptr;
+    int cur = 0;  // buffer index written by the last step (0 before any step)
+};
+
+struct Parent {
+    const Level* P = nullptr;
+    const float *f_new = nullptr, *rho_new = nullptr, *vel_new = nullptr;
+    const float *f_old = nullptr, *rho_old = nullptr, *vel_old = nullptr;
+    const float *g_new = nullptr, *T_new = nullptr, *g_old = nullptr, *T_old = nullptr;
+};
+
+}  // namespace
+
+struct lb_ctx {
+    int device = 0;
+    lb_config cfg{};
+    hipStream_t stream = nullptr;
+    const Kern* k = &kStrict;
+    std::vector<Level> levels;
+    std::vector<void*> allocs;
+    int64_t bytes = 0;
+    std::string err;
+    // mesh
+    int64_t n_tri = 0;
+    float* mesh[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    float* maps[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* fpart = nullptr;
+    double* spart = nullptr;
+    int n_stat_groups = 0;
+};
+
+namespace {
+
+int fail(lb_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                                 \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return fail(ctx, LB_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+template <class T>
+int dalloc(lb_ctx* c, T** p, size_t n) {
+    *p = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(c, LB_ENOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
+    c->allocs.push_back((void*)*p);
+    c->bytes += (int64_t)bytes;
+    return LB_OK;
+}
+
+template <class T>
+int h2d(lb_ctx* c, T* dst, const T* src, size_t n) {
+    hipError_t e = hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(c, LB_EHIP, "hipMemcpy H2D: %s", hipGetErrorString(e));
+    return LB_OK;
+}
+
+float half_to_float(uint16_t h) {
+    int s = (h >> 15) & 1, e = (h >> 10) & 31, m = h & 1023;
+    float v;
+    if (e == 0) v = std::ldexp((float)m, -24);
+    else if (e == 31) v = m ? NAN : INFINITY;
+    else v = std::ldexp((float)(m | 1024), e - 25);
+    return s ? -v : v;
+}
+
+inline size_t fidx(int64_t b, int k, int c) { return ((size_t)b * Q + k) * NC + c; }
+
+#define RC(expr)                \
+    do {                        \
+        int rc_ = (expr);       \
+        if (rc_ != LB_OK) return rc_; \
+    } while (0)
+
+// Build the Bouzidi link list of one level (src/geometry/bouzidi_kernel.jl:35-89):
+// link (cell x, direction k) with q_min < q <= 1 corrects slot (x, opp k).
+int build_links(lb_ctx* c, Level& L, const lb_level_desc* d) {
+    const int64_t nbc = d->n_boundary_cells;
+    if (nbc <= 0) return LB_OK;
+    if (!d->bouzidi_cell_block || !d->bouzidi_cell_x || !d->bouzidi_cell_y || !d->bouzidi_cell_z ||
+        (!d->bouzidi_q_dense && !d->bouzidi_q_cells))
+        return fail(c, LB_EINVAL, "level %d: Bouzidi cell list or q map missing", d->level_id);
+    const float qmin = c->cfg.q_min_threshold;
+    const int64_t nb = d->n_blocks;
+    std::vector<int64_t> dst, s1, s2;
+    std::vector<float> ca, cb;
+    for (int64_t i = 0; i < nbc; i++) {
+        int b = d->bouzidi_cell_block[i], x = d->bouzidi_cell_x[i], y = d->bouzidi_cell_y[i],
+            z = d->bouzidi_cell_z[i];
+        if (b < 1 || b > nb || x < 1 || x > 8 || y < 1 || y > 8 || z < 1 || z > 8)
+            return fail(c, LB_EINVAL, "level %d: Bouzidi cell %lld out of range", d->level_id, (long long)i);
+        int cl = (x - 1) + 8 * (y - 1) + 64 * (z - 1);
+        for (int k = 0; k < Q; k++) {
+            uint16_t bits = d->bouzidi_q_cells
+                                ? d->bouzidi_q_cells[i * Q + k]
+                                : d->bouzidi_q_dense[(size_t)cl + 512 * (size_t)(b - 1) + 512 * (size_t)nb * k];
+            float q = half_to_float(bits);
+            if (!(q > qmin && q <= 1.0f)) continue;
+            int ok = 26 - k;
+            int cx = k % 3 - 1, cy = (k / 3) % 3 - 1, cz = k / 9 - 1;
+            int64_t sA = (int64_t)fidx(b - 1, k, cl);
+            int64_t sB;
+            float a, bb;
+            if (q < 0.5f) {
+                int nx = x - cx, ny = y - cy, nz = z - cz;
+                if (nx >= 1 && nx <= 8 && ny >= 1 && ny <= 8 && nz >= 1 && nz <= 8) {
+                    sB = (int64_t)fidx(b - 1, k, (nx - 1) + 8 * (ny - 1) + 64 * (nz - 1));
+                } else {
+                    int ox = nx < 1 ? -1 : (nx > 8 ? 1 : 0), oy = ny < 1 ? -1 : (ny > 8 ? 1 : 0),
+                        oz = nz < 1 ? -1 : (nz > 8 ? 1 : 0);
+                    int dir = (ox + 1) + 3 * (oy + 1) + 9 * (oz + 1);
+                    int nbb = d->neighbor_table[(size_t)(b - 1) + (size_t)nb * dir];
+                    if (nbb > 0) {
+                        int wx = nx < 1 ? nx + 8 : (nx > 8 ? nx - 8 : nx);
+                        int wy = ny < 1 ? ny + 8 : (ny > 8 ? ny - 8 : ny);
+                        int wz = nz < 1 ? nz + 8 : (nz > 8 ? nz - 8 : nz);
+                        sB = (int64_t)fidx(nbb - 1, k, (wx - 1) + 8 * (wy - 1) + 64 * (wz - 1));
+                    } else {
+                        sB = sA;
+                    }
+                }
+                float c1 = 2.0f * q;
+                a = c1;
+                bb = 1.0f - c1;
+            } else {
+                sB = (int64_t)fidx(b - 1, ok, cl);
+                float t2 = 2.0f * q;
+                float inv = 1.0f / t2;
+                a = inv;
+                bb = (2.0f * q - 1.0f) * inv;
+            }
+            dst.push_back((int64_t)fidx(b - 1, ok, cl));
+            s1.push_back(sA);
+            s2.push_back(sB);
+            ca.push_back(a);
+            cb.push_back(bb);
+        }
+    }
+    L.nlinks = (int64_t)dst.size();
+    if (L.nlinks == 0) return LB_OK;
+    size_t n = (size_t)L.nlinks;
+    RC(dalloc(c, &L.ldst, n));
+    RC(dalloc(c, &L.ls1, n));
+    RC(dalloc(c, &L.ls2, n));
+    RC(dalloc(c, &L.la, n));
+    RC(dalloc(c, &L.lb, n));
+    RC(dalloc(c, &L.ltmp, n));
+    RC(h2d(c, L.ldst, dst.data(), n));
+    RC(h2d(c, L.ls1, s1.data(), n));
+    RC(h2d(c, L.ls2, s2.data(), n));
+    RC(h2d(c, L.la, ca.data(), n));
+    RC(h2d(c, L.lb, cb.data(), n));
+    return LB_OK;
+}
+
+lbk::BouzidiLinks links_of(const Level& L) {
+    lbk::BouzidiLinks l;
+    l.n = L.nlinks;
+    l.dst = L.ldst;
+    l.s1 = L.ls1;
+    l.s2 = L.ls2;
+    l.a = L.la;
+    l.b = L.lb;
+    l.tmp = L.ltmp;
+    return l;
+}
+
+// perform_timestep_v2! + recursive_step! for one level (timestep.jl:1-159,
+// solver_control.jl:6-197).
+int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, float tw, const lb_step_params* p) {
+    if (li >= (int)c->levels.size()) return LB_OK;
+    Level& L = c->levels[(size_t)li];
+    const int par = (int)(t_sub & 1);
+    const int out = 1 - par;
+    const bool ddf = c->cfg.ddf_enabled && L.g[0] != nullptr;
+    const bool has_children = li + 1 < (int)c->levels.size();
+    if (L.nb > 0) {
+        lbk::CollideArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.f_in = L.f[par];
+        a.f_out = L.f[out];
+        a.vel_in = L.vel[par];
+        a.vel_out = L.vel[out];
+        a.rho_out = L.rho[out];
+        a.obst = L.obst;
+        a.sponge = L.sponge;
+        a.wall_dist = L.wall_dist;
+        a.nbr = L.nbr;
+        a.borg = L.borg;
+        a.nb = L.nb;
+        const int scale = 1 << (L.id - 1);
+        a.nxg = c->cfg.domain_nx * scale;
+        a.nyg = c->cfg.domain_ny * scale;
+        a.nzg = c->cfg.domain_nz * scale;
+        a.tau = L.tau;
+        a.u_inlet = u;
+        a.c_wale = p->c_wale;
+        a.nu_sgs_bg = p->nu_sgs_background;
+        a.inlet_turb = p->inlet_turbulence;
+        a.omega_bulk = p->omega_bulk;
+        a.omega3 = p->omega_3;
+        a.omega4 = p->omega_4;
+        a.lambda = p->lambda_param;
+        a.adaptive = p->adaptive_omega4;
+        a.limiter = p->limiter;
+        a.comp = p->compressible;
+        a.wall_model = (p->wall_model && L.wall_dist) ? 1 : 0;
+        a.sponge_blend = p->sponge_blend_distributions;
+        a.seed = (int32_t)(t_sub % 1000000);
+        a.is_l1 = pv == nullptr ? 1 : 0;
+        a.tw = tw;
+        a.temporal = c->cfg.temporal_interpolation ? 1 : 0;
+        if (pv) {
+            a.pf_new = pv->f_new;
+            a.prho_new = pv->rho_new;
+            a.pvel_new = pv->vel_new;
+            a.pf_old = pv->f_old;
+            a.prho_old = pv->rho_old;
+            a.pvel_old = pv->vel_old;
+            a.pgrid = pv->P->grid;
+            a.pdx = pv->P->pd[0];
+            a.pdy = pv->P->pd[1];
+            a.pdz = pv->P->pd[2];
+            a.pnb = pv->P->nb;
+            a.tau_parent = pv->P->tau;
+        } else {
+            a.tau_parent = 0.5f;
+        }
+        HIPCHK(c, c->k->collide(a, c->stream));
+        if (L.nlinks > 0) HIPCHK(c, c->k->bouzidi(links_of(L), L.f[out], c->stream));
+        if (ddf) {
+            lbk::ThermalArgs t;
+            std::memset(&t, 0, sizeof t);
+            t.g_in = L.g[par];
+            t.g_out = L.g[out];
+            t.vel = L.vel[out];
+            t.T_out = L.T[out];
+            t.obst = L.obst;
+            t.sponge = L.sponge;
+            t.nbr = L.nbr;
+            t.borg = L.borg;
+            t.nb = L.nb;
+            t.nxg = a.nxg;
+            t.nyg = a.nyg;
+            t.nzg = a.nzg;
+            t.tau_g = L.tau_g;
+            t.u_inlet = u;
+            t.T_inlet = p->T_inlet;
+            t.T_wall = p->T_wall;
+            t.adiabatic = p->wall_bc_adiabatic;
+            t.sponge_blend = p->sponge_blend_distributions;
+            t.is_l1 = a.is_l1;
+            t.tw = tw;
+            t.temporal = a.temporal;
+            if (pv) {
+                t.pg_new = pv->g_new;
+                t.pT_new = pv->T_new;
+                t.pvel_new = pv->vel_new;
+                t.pg_old = pv->g_old;
+                t.pT_old = pv->T_old;
+                t.pvel_old = pv->vel_old;
+                t.pgrid = pv->P->grid;
+                t.pdx = pv->P->pd[0];
+                t.pdy = pv->P->pd[1];
+                t.pdz = pv->P->pd[2];
+                t.pnb = pv->P->nb;
+                t.tau_g_parent = pv->P->tau_g;
+            } else {
+                t.tau_g_parent = 0.5f;
+            }
+            HIPCHK(c, c->k->thermal(t, c->stream));
+            if (L.nlinks > 0) HIPCHK(c, c->k->bouzidi(links_of(L), L.g[out], c->stream));
+        }
+    }
+    L.cur = out;
+    if (has_children) {
+        Parent ch;
+        ch.P = &L;
+        ch.f_new = L.f[out];
+        ch.rho_new = L.rho[out];
+        ch.vel_new = L.vel[out];
+        ch.f_old = L.f[par];
+        ch.rho_old = L.rho[par];
+        ch.vel_old = L.vel[par];
+        if (ddf) {
+            ch.g_new = L.g[out];
+            ch.T_new = L.T[out];
+            ch.g_old = L.g[par];
+            ch.T_old = L.T[par];
+        }
+        RC(step_level(c, li + 1, 2 * t_sub, &ch, u, 0.0f, p));
+        RC(step_level(c, li + 1, 2 * t_sub + 1, &ch, u, 0.5f, p));
+    }
+    return LB_OK;
+}
+
+int check_ctx(lb_ctx* c) {
+    if (!c) return LB_EINVAL;
+    for (size_t i = 0; i < c->levels.size(); i++)
+        if (!c->levels[i].uploaded) return fail(c, LB_ESTATE, "level %zu not uploaded", i + 1);
+    return LB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lb_last_error(const lb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+const char* lb_version(void) { return "ludwig_hip 0.1 gfx950"; }
+
+int lb_create(lb_ctx** out, int device, const lb_config* cfg) {
+    if (!out || !cfg) return LB_EINVAL;
+    *out = nullptr;
+    if (cfg->n_levels < 1 || cfg->n_levels > 16 || cfg->domain_nx < 8 || cfg->domain_ny < 8 || cfg->domain_nz < 8)
+        return LB_EINVAL;
+    if (cfg->storage != LB_STORAGE_FP32) return LB_EINVAL;
+    lb_ctx* c = new lb_ctx();
+    c->device = device;
+    c->cfg = *cfg;
+    c->k = cfg->strict_fp ? &kStrict : &kFast;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        delete c;
+        return LB_EHIP;
+    }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return LB_EHIP;
+    }
+    c->levels.resize((size_t)cfg->n_levels);
+    *out = c;
+    return LB_OK;
+}
+
+void lb_destroy(lb_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
+    if (!c || !d) return LB_EINVAL;
+    if (d->level_id < 1 || d->level_id > (int)c->levels.size())
+        return fail(c, LB_EINVAL, "level_id %d out of range", d->level_id);
+    Level& L = c->levels[(size_t)d->level_id - 1];
+    if (L.uploaded) return fail(c, LB_ESTATE, "level %d already uploaded", d->level_id);
+    if (d->level_id > 1 && !c->levels[(size_t)d->level_id - 2].uploaded)
+        return fail(c, LB_ESTATE, "levels must be uploaded in order");
+    const int32_t nb = d->n_blocks;
+    if (nb < 0 || (nb > 0 && (!d->map_x || !d->map_y || !d->map_z || !d->neighbor_table || !d->obstacle ||
+                             !d->sponge || !d->block_pointer)))
+        return fail(c, LB_EINVAL, "level %d: missing arrays", d->level_id);
+    if (d->ptr_dim_x < 0 || d->ptr_dim_y < 0 || d->ptr_dim_z < 0)
+        return fail(c, LB_EINVAL, "level %d: bad block_pointer dims", d->level_id);
+    HIPCHK(c, hipSetDevice(c->device));
+    L.id = d->level_id;
+    L.nb = nb;
+    L.tau = d->tau;
+    L.tau_g = d->tau_g;
+    L.dx = d->dx;
+    const size_t n4 = (size_t)nb * NC, n5 = n4 * Q;
+    for (int i = 0; i < 2; i++) {
+        RC(dalloc(c, &L.f[i], n5));
+        RC(dalloc(c, &L.rho[i], n4));
+        RC(dalloc(c, &L.vel[i], n4 * 3));
+        if (c->cfg.ddf_enabled) {
+            RC(dalloc(c, &L.g[i], n5));
+            RC(dalloc(c, &L.T[i], n4));
+        }
+    }
+    RC(dalloc(c, &L.obst, n4));
+    RC(dalloc(c, &L.sponge, n4));
+    if (d->wall_dist) RC(dalloc(c, &L.wall_dist, n4));
+    RC(dalloc(c, &L.nbr, (size_t)nb * Q));
+    RC(dalloc(c, &L.borg, (size_t)nb * 3));
+    const size_t ng = (size_t)d->ptr_dim_x * d->ptr_dim_y * d->ptr_dim_z;
+    RC(dalloc(c, &L.grid, ng));
+    L.pd[0] = d->ptr_dim_x;
+    L.pd[1] = d->ptr_dim_y;
+    L.pd[2] = d->ptr_dim_z;
+    if (nb > 0) {
+        std::vector<int32_t> nbr((size_t)nb * Q), borg((size_t)nb * 3);
+        for (int32_t b = 0; b < nb; b++) {
+            for (int k = 0; k < Q; k++) {
+                int v = d->neighbor_table[(size_t)b + (size_t)nb * k];
+                if (v < 0 || v > nb) return fail(c, LB_EINVAL, "level %d: neighbour table entry out of range", L.id);
+                nbr[(size_t)b * Q + k] = v - 1;
+            }
+            if (nbr[(size_t)b * Q + 13] != b)
+                return fail(c, LB_EINVAL, "level %d: neighbour table centre entry of block %d is not itself", L.id,
+                            b + 1);
+            borg[(size_t)b * 3 + 0] = (d->map_x[b] - 1) * 8;
+            borg[(size_t)b * 3 + 1] = (d->map_y[b] - 1) * 8;
+            borg[(size_t)b * 3 + 2] = (d->map_z[b] - 1) * 8;
+        }
+        RC(h2d(c, L.nbr, nbr.data(), nbr.size()));
+        RC(h2d(c, L.borg, borg.data(), borg.size()));
+        RC(h2d(c, L.obst, d->obstacle, n4));
+        RC(h2d(c, L.sponge, d->sponge, n4));
+        if (d->wall_dist) RC(h2d(c, L.wall_dist, d->wall_dist, n4));
+    }
+    if (ng > 0) {
+        std::vector<int32_t> grid(ng);
+        for (size_t i = 0; i < ng; i++) {
+            int v = d->block_pointer[i];
+            if (v < 0 || v > nb) return fail(c, LB_EINVAL, "level %d: block_pointer entry out of range", L.id);
+            grid[i] = v - 1;
+        }
+        RC(h2d(c, L.grid, grid.data(), ng));
+    }
+    RC(build_links(c, L, d));
+    L.uploaded = true;
+    return LB_OK;
+}
+
+int lb_init_equilibrium(lb_ctx* c, float T0) {
+    RC(check_ctx(c));
+    HIPCHK(c, hipSetDevice(c->device));
+    for (auto& L : c->levels) {
+        HIPCHK(c, c->k->init_level(L.f[0], L.f[1], L.rho[0], L.rho[1], L.vel[0], L.vel[1], L.nb, c->stream));
+        if (c->cfg.ddf_enabled) HIPCHK(c, c->k->init_thermal(L.g[0], L.g[1], L.T[0], L.T[1], L.nb, T0, c->stream));
+        L.cur = 0;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LB_OK;
+}
+
+int lb_step_batch(lb_ctx* c, int64_t t_start, int32_t n_steps, float u_curr, const lb_step_params* p) {
+    RC(check_ctx(c));
+    if (!p || n_steps < 0 || t_start < 0) return fail(c, LB_EINVAL, "bad step arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    for (int64_t t = t_start; t < t_start + n_steps; t++) RC(step_level(c, 0, t, nullptr, u_curr, 0.0f, p));
+    return LB_OK;
+}
+
+int lb_synchronize(lb_ctx* c) {
+    if (!c) return LB_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LB_OK;
+}
+
+int lb_mesh_upload(lb_ctx* c, int64_t n, const float* cx, const float* cy, const float* cz, const float* nx,
+                   const float* ny, const float* nz, const float* areas) {
+    if (!c || n < 0 || (n > 0 && (!cx || !cy || !cz || !nx || !ny || !nz || !areas))) return LB_EINVAL;
+    if (c->n_tri != 0) return fail(c, LB_ESTATE, "mesh already uploaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    const float* src[7] = {cx, cy, cz, nx, ny, nz, areas};
+    for (int i = 0; i < 7; i++) {
+        RC(dalloc(c, &c->mesh[i], (size_t)n));
+        if (n > 0) RC(h2d(c, c->mesh[i], src[i], (size_t)n));
+    }
+    for (int i = 0; i < 4; i++) RC(dalloc(c, &c->maps[i], (size_t)n));
+    int groups = (int)((n + 255) / 256);
+    RC(dalloc(c, &c->fpart, (size_t)std::max(groups, 1) * 9));
+    c->n_tri = n;
+    return LB_OK;
+}
+
+int lb_forces(lb_ctx* c, const lb_force_params* fp, double* out, float* p_map, float* sx_map, float* sy_map,
+              float* sz_map) {
+    RC(check_ctx(c));
+    if (!fp || !out) return fail(c, LB_EINVAL, "null argument");
+    if (c->n_tri <= 0) return fail(c, LB_ESTATE, "no mesh uploaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    const Level& L = c->levels.back();
+    lbk::ForceArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.rho = L.rho[L.cur];
+    a.vel = L.vel[0];  // level.vel, as surface.jl:784 reads it
+    a.obst = L.obst;
+    a.grid = L.grid;
+    a.gdx = L.pd[0];
+    a.gdy = L.pd[1];
+    a.gdz = L.pd[2];
+    a.dx = (float)L.dx;
+    a.n_tri = c->n_tri;
+    a.cx = c->mesh[0]; a.cy = c->mesh[1]; a.cz = c->mesh[2];
+    a.nx = c->mesh[3]; a.ny = c->mesh[4]; a.nz = c->mesh[5];
+    a.area = c->mesh[6];
+    a.off_x = (float)fp->mesh_offset[0];
+    a.off_y = (float)fp->mesh_offset[1];
+    a.off_z = (float)fp->mesh_offset[2];
+    const float ps = (float)(fp->rho_physical * fp->velocity_scale * fp->velocity_scale);
+    a.pressure_scale = ps;
+    a.stress_scale = ps;
+    a.tau_mol = L.tau;
+    a.search_radius = fp->search_radius;
+    a.c_wale = fp->c_wale;
+    a.nu_sgs_bg = fp->nu_sgs_background;
+    a.ref_x = (float)fp->moment_center[0];
+    a.ref_y = (float)fp->moment_center[1];
+    a.ref_z = (float)fp->moment_center[2];
+    a.p_map = c->maps[0];
+    a.sx_map = c->maps[1];
+    a.sy_map = c->maps[2];
+    a.sz_map = c->maps[3];
+    a.partials = c->fpart;
+    const int groups = (int)((c->n_tri + 255) / 256);
+    HIPCHK(c, c->k->forces(a, groups, nullptr, c->stream));
+    std::vector<double> part((size_t)groups * 9);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->fpart, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (p_map) HIPCHK(c, hipMemcpyAsync(p_map, c->maps[0], (size_t)c->n_tri * 4, hipMemcpyDeviceToHost, c->stream));
+    if (sx_map) HIPCHK(c, hipMemcpyAsync(sx_map, c->maps[1], (size_t)c->n_tri * 4, hipMemcpyDeviceToHost, c->stream));
+    if (sy_map) HIPCHK(c, hipMemcpyAsync(sy_map, c->maps[2], (size_t)c->n_tri * 4, hipMemcpyDeviceToHost, c->stream));
+    if (sz_map) HIPCHK(c, hipMemcpyAsync(sz_map, c->maps[3], (size_t)c->n_tri * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int g = 0; g < groups; g++)
+        for (int j = 0; j < 9; j++) s[j] += part[(size_t)g * 9 + j];
+    double Fxp = s[0], Fyp = s[1], Fzp = s[2], Fxv = s[3], Fyv = s[4], Fzv = s[5], Mx = s[6], My = s[7], Mz = s[8];
+    // integrate_surface_forces! (surface.jl:873-910)
+    if (fp->symmetric) {
+        Fxp *= 2.0; Fzp *= 2.0; Fxv *= 2.0; Fzv *= 2.0; My *= 2.0;
+        Fyp = 0.0; Fyv = 0.0; Mx = 0.0; Mz = 0.0;
+    }
+    out[LB_F_FXP] = Fxp; out[LB_F_FYP] = Fyp; out[LB_F_FZP] = Fzp;
+    out[LB_F_FXV] = Fxv; out[LB_F_FYV] = Fyv; out[LB_F_FZV] = Fzv;
+    out[LB_F_MX] = Mx; out[LB_F_MY] = My; out[LB_F_MZ] = Mz;
+    out[LB_F_FX] = Fxp + Fxv; out[LB_F_FY] = Fyp + Fyv; out[LB_F_FZ] = Fzp + Fzv;
+    double q_inf = 0.5 * fp->rho_ref * fp->u_ref * fp->u_ref;
+    double F_ref = q_inf * fp->area_ref;
+    double M_ref = F_ref * fp->chord_ref;
+    out[LB_F_CD] = out[LB_F_CL] = out[LB_F_CS] = 0.0;
+    out[LB_F_CMX] = out[LB_F_CMY] = out[LB_F_CMZ] = 0.0;
+    if (F_ref > 1e-10) {
+        out[LB_F_CD] = out[LB_F_FX] / F_ref;
+        out[LB_F_CL] = out[LB_F_FZ] / F_ref;
+        out[LB_F_CS] = out[LB_F_FY] / F_ref;
+    }
+    if (M_ref > 1e-10) {
+        out[LB_F_CMX] = Mx / M_ref;
+        out[LB_F_CMY] = My / M_ref;
+        out[LB_F_CMZ] = Mz / M_ref;
+    }
+    return LB_OK;
+}
+
+int lb_download_level(lb_ctx* c, int level_id, int field, float* dst) {
+    RC(check_ctx(c));
+    if (!dst || level_id < 1 || level_id > (int)c->levels.size()) return fail(c, LB_EINVAL, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    const Level& L = c->levels[(size_t)level_id - 1];
+    const size_t n4 = (size_t)L.nb * NC;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    switch (field) {
+        case LB_FIELD_F:
+        case LB_FIELD_G: {
+            const float* src = field == LB_FIELD_F ? L.f[L.cur] : L.g[L.cur];
+            if (!src) return fail(c, LB_EINVAL, "field not allocated");
+            std::vector<float> tmp(n4 * Q);
+            HIPCHK(c, hipMemcpy(tmp.data(), src, tmp.size() * 4, hipMemcpyDeviceToHost));
+            for (int64_t b = 0; b < L.nb; b++)
+                for (int k = 0; k < Q; k++)
+                    std::memcpy(dst + (size_t)k * n4 + (size_t)b * NC, tmp.data() + fidx(b, k, 0), NC * 4);
+            return LB_OK;
+        }
+        case LB_FIELD_RHO:
+        case LB_FIELD_T: {
+            const float* src = field == LB_FIELD_RHO ? L.rho[L.cur] : L.T[L.cur];
+            if (!src) return fail(c, LB_EINVAL, "field not allocated");
+            HIPCHK(c, hipMemcpy(dst, src, n4 * 4, hipMemcpyDeviceToHost));
+            return LB_OK;
+        }
+        case LB_FIELD_VEL: {
+            std::vector<float> tmp(n4 * 3);
+            HIPCHK(c, hipMemcpy(tmp.data(), L.vel[L.cur], tmp.size() * 4, hipMemcpyDeviceToHost));
+            for (int64_t b = 0; b < L.nb; b++)
+                for (int d = 0; d < 3; d++)
+                    std::memcpy(dst + (size_t)d * n4 + (size_t)b * NC, tmp.data() + ((size_t)b * 3 + d) * NC, NC * 4);
+            return LB_OK;
+        }
+        default:
+            return fail(c, LB_EINVAL, "unknown field %d", field);
+    }
+}
+
+// Overwrite the current state of a level (both halves of the ping-pong pair,
+// so the next step reads it whatever its parity).
+int lb_upload_level(lb_ctx* c, int level_id, int field, const float* src) {
+    RC(check_ctx(c));
+    if (!src || level_id < 1 || level_id > (int)c->levels.size()) return fail(c, LB_EINVAL, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    Level& L = c->levels[(size_t)level_id - 1];
+    const size_t n4 = (size_t)L.nb * NC;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<float> tmp;
+    float* const* pair;
+    size_t n;
+    switch (field) {
+        case LB_FIELD_F:
+        case LB_FIELD_G:
+            pair = field == LB_FIELD_F ? L.f : L.g;
+            if (!pair[0]) return fail(c, LB_EINVAL, "field not allocated");
+            tmp.resize(n4 * Q);
+            for (int64_t b = 0; b < L.nb; b++)
+                for (int k = 0; k < Q; k++)
+                    std::memcpy(tmp.data() + fidx(b, k, 0), src + (size_t)k * n4 + (size_t)b * NC, NC * 4);
+            n = tmp.size();
+            break;
+        case LB_FIELD_RHO:
+        case LB_FIELD_T:
+            pair = field == LB_FIELD_RHO ? L.rho : L.T;
+            if (!pair[0]) return fail(c, LB_EINVAL, "field not allocated");
+            tmp.assign(src, src + n4);
+            n = n4;
+            break;
+        case LB_FIELD_VEL:
+            pair = L.vel;
+            tmp.resize(n4 * 3);
+            for (int64_t b = 0; b < L.nb; b++)
+                for (int d = 0; d < 3; d++)
+                    std::memcpy(tmp.data() + ((size_t)b * 3 + d) * NC, src + (size_t)d * n4 + (size_t)b * NC, NC * 4);
+            n = tmp.size();
+            break;
+        default:
+            return fail(c, LB_EINVAL, "unknown field %d", field);
+    }
+    for (int i = 0; i < 2; i++) HIPCHK(c, hipMemcpy(pair[i], tmp.data(), n * 4, hipMemcpyHostToDevice));
+    return LB_OK;
+}
+
+int lb_stats(lb_ctx* c, int level_id, double* out) {
+    RC(check_ctx(c));
+    if (!out || level_id < 1 || level_id > (int)c->levels.size()) return fail(c, LB_EINVAL, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    const Level& L = c->levels[(size_t)level_id - 1];
+    const int groups = std::max(1, std::min<int>(L.nb, 1024));
+    if (c->n_stat_groups < groups) {
+        RC(dalloc(c, &c->spart, (size_t)1024 * 7));
+        c->n_stat_groups = 1024;
+    }
+    for (int i = 0; i < 8; i++) out[i] = 0;
+    if (L.nb == 0) return LB_OK;
+    HIPCHK(c, c->k->stats(L.f[L.cur], L.rho[L.cur], L.vel[L.cur], L.obst, L.nb, c->spart, groups, nullptr, c->stream));
+    std::vector<double> part((size_t)groups * 7);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->spart, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double mn = 1e300, mx = -1e300;
+    for (int g = 0; g < groups; g++) {
+        const double* q = &part[(size_t)g * 7];
+        out[0] += q[0];
+        out[1] += q[1];
+        mn = std::min(mn, q[2]);
+        mx = std::max(mx, q[3]);
+        out[4] += q[4];
+        out[5] += q[5];
+        out[6] += q[6];
+    }
+    out[2] = mn;
+    out[3] = mx;
+    return LB_OK;
+}
+
+int64_t lb_device_bytes(const lb_ctx* c) { return c ? c->bytes : 0; }
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+// lb_kernels.h -- argument blocks and launchers of the gfx950 kernels.
+//
+// lb_kernels.hip is compiled twice: once with -ffp-contract=off into namespace
+// lbk_strict (every float operation rounded as written, so results compare
+// bit for bit with a plain C restatement) and once with FMA contraction into
+// lbk_fast.  The API layer (lb_api.hip) picks one per context.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace lbk {
+
+constexpr int BS = 8;
+constexpr int NC = 512;  // cells per block
+constexpr int Q = 27;
+
+// Storage of one level on the device (block-major SoA):
+//   f[b][k][c]   c = x + 8y + 64z (0-based local cell), float
+//   rho[b][c], vel[b][d][c], obst[b][c], sponge[b][c], wall_dist[b][c]
+//   nbr[b][27]   0-based neighbour block, -1 absent (dir as k)
+//   borg[b][3]   global 1-based cell coordinate of local (0,0,0) minus 1
+//   grid         dense block pointer (max active coords), 0-based ids, -1 absent
+struct CollideArgs {
+    const float* f_in;
+    float* f_out;
+    const float* vel_in;
+    float* vel_out;
+    float* rho_out;
+    const uint8_t* obst;
+    const float* sponge;
+    const float* wall_dist;  // may be null when wall_model == 0
+    const int32_t* nbr;
+    const int32_t* borg;
+    int32_t nb;
+    int32_t nxg, nyg, nzg;
+    float tau;
+    float u_inlet;
+    float c_wale, nu_sgs_bg, inlet_turb;
+    float omega_bulk, omega3, omega4, lambda;
+    int32_t adaptive, limiter, comp, wall_model, sponge_blend;
+    int32_t seed;
+    // parent (level > 1)
+    int32_t is_l1;
+    const float* pf_new;
+    const float* prho_new;
+    const float* pvel_new;
+    const float* pf_old;
+    const float* prho_old;
+    const float* pvel_old;
+    const int32_t* pgrid;
+    int32_t pdx, pdy, pdz;
+    int32_t pnb;
+    float tau_parent;
+    float tw;
+    int32_t temporal;
+};
+
+struct ThermalArgs {
+    const float* g_in;
+    float* g_out;
+    const float* vel;  // the flow step's vel_out
+    float* T_out;
+    const uint8_t* obst;
+    const float* sponge;
+    const int32_t* nbr;
+    const int32_t* borg;
+    int32_t nb;
+    int32_t nxg, nyg, nzg;
+    float tau_g;
+    float u_inlet, T_inlet, T_wall;
+    int32_t adiabatic, sponge_blend;
+    int32_t is_l1;
+    const float* pg_new;
+    const float* pT_new;
+    const float* pvel_new;
+    const float* pg_old;
+    const float* pT_old;
+    const float* pvel_old;
+    const int32_t* pgrid;
+    int32_t pdx, pdy, pdz;
+    int32_t pnb;
+    float tau_g_parent;
+    float tw;
+    int32_t temporal;
+};
+
+// Bouzidi link list: corrected slot dst = a*f[s1] + b*f[s2], all indices
+// into one population buffer (block-major layout).
+struct BouzidiLinks {
+    int64_t n;
+    const int64_t* dst;
+    const int64_t* s1;
+    const int64_t* s2;
+    const float* a;
+    const float* b;
+    float* tmp;
+};
+
+struct ForceArgs {
+    // finest level
+    const float* rho;
+    const float* vel;
+    const uint8_t* obst;
+    const int32_t* grid;
+    int32_t gdx, gdy, gdz;
+    float dx;
+    // mesh
+    int64_t n_tri;
+    const float* cx;
+    const float* cy;
+    const float* cz;
+    const float* nx;
+    const float* ny;
+    const float* nz;
+    const float* area;
+    float off_x, off_y, off_z;
+    float pressure_scale, stress_scale, tau_mol;
+    int32_t search_radius;
+    float c_wale, nu_sgs_bg;
+    float ref_x, ref_y, ref_z;
+    // outputs
+    float* p_map;
+    float* sx_map;
+    float* sy_map;
+    float* sz_map;
+    double* partials;  // [n_groups][9]
+};
+
+}  // namespace lbk
+
+#define LBK_DECLARE(NS)                                                                                  \
+    namespace NS {                                                                                       \
+    hipError_t collide(const lbk::CollideArgs& a, hipStream_t s);                                        \
+    hipError_t thermal(const lbk::ThermalArgs& a, hipStream_t s);                                        \
+    hipError_t bouzidi(const lbk::BouzidiLinks& l, float* f, hipStream_t s);                             \
+    hipError_t init_level(float* f0, float* f1, float* rho0, float* rho1, float* v0, float* v1, int32_t nb, \
+                          hipStream_t s);                                                                \
+    hipError_t init_thermal(float* g0, float* g1, float* T0, float* T1, int32_t nb, float Tinit,         \
+                            hipStream_t s);                                                              \
+    hipError_t forces(const lbk::ForceArgs& a, int n_groups, double* out9, hipStream_t s);               \
+    hipError_t stats(const float* f, const float* rho, const float* vel, const uint8_t* obst, int32_t nb, \
+                     double* partials, int n_groups, double* out, hipStream_t s);                        \
+    }
+
+LBK_DECLARE(lbk_strict)
+LBK_DECLARE(lbk_fast)

@@ -1,0 +1,951 @@
+// lb_kernels.hip -- gfx950 kernels of the OpenLUDWIG time step.
+//
+// One work-item per cell, 256 work-items (4 wave64) per work-group covering
+// half of an 8^3 block; a wave is one 8x8 z-plane, so every population load
+// and store of a wave is a contiguous 256-byte row of the block-major SoA
+// layout f[b][k][c].  The pull for a cell whose source lies in another block
+// goes through that block's neighbour-table row, staged in LDS; the centre
+// entry of the row is the block itself, so in-block and cross-block pulls are
+// one load instruction with a per-lane block index.  Sources with no
+// neighbour block fall to the boundary cascade (inlet/outlet equilibrium,
+// y/z mirror, coarse-to-fine interpolation), a non-inlined cold path.
+//
+// Operation order follows the reference kernels statement by statement (see
+// the citations), so that with -ffp-contract=off the results equal a plain
+// C restatement of them bit for bit.
+//
+// This file is compiled twice; LBK_NS names the namespace (lbk_strict /
+// lbk_fast).
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "lb_kernels.h"
+
+#ifndef LBK_NS
+#error "LBK_NS must be defined"
+#endif
+
+namespace LBK_NS {
+using namespace lbk;
+
+// ---------------------------------------------------------------------------
+// D3Q27 lattice (src/collision/lattice.jl:5-56, timestep.jl:161-179):
+// k = (cx+1) + 3(cy+1) + 9(cz+1), 0-based; rest = 13; opp(k) = 26-k.
+__host__ __device__ constexpr int CX(int k) { return k % 3 - 1; }
+__host__ __device__ constexpr int CY(int k) { return (k / 3) % 3 - 1; }
+__host__ __device__ constexpr int CZ(int k) { return k / 9 - 1; }
+__host__ __device__ constexpr int OPP(int k) { return 26 - k; }
+__host__ __device__ constexpr int MIRY(int k) { return k - 6 * CY(k); }
+__host__ __device__ constexpr int MIRZ(int k) { return k - 18 * CZ(k); }
+__host__ __device__ constexpr float WK(int k) {
+    return (CX(k) * CX(k) + CY(k) * CY(k) + CZ(k) * CZ(k)) == 0   ? 8.0f / 27.0f
+           : (CX(k) * CX(k) + CY(k) * CY(k) + CZ(k) * CZ(k)) == 1 ? 2.0f / 27.0f
+           : (CX(k) * CX(k) + CY(k) * CY(k) + CZ(k) * CZ(k)) == 2 ? 1.0f / 54.0f
+                                                                  : 1.0f / 216.0f;
+}
+
+__device__ __forceinline__ size_t fidx(int b, int k, int c) { return ((size_t)b * Q + k) * NC + c; }
+
+// Equilibria (src/collision/physics_utils.jl:19-82)
+__device__ __forceinline__ float feq2(float rho, float ux, float uy, float uz, float w, float cx, float cy, float cz) {
+    float cu = cx * ux + cy * uy + cz * uz;
+    float usq = ux * ux + uy * uy + uz * uz;
+    return rho * w * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * usq);
+}
+__device__ __forceinline__ float feq4(float rho, float ux, float uy, float uz, float w, float cx, float cy, float cz) {
+    float cu = cx * ux + cy * uy + cz * uz;
+    float usq = ux * ux + uy * uy + uz * uz;
+    float cu2 = cu * cu;
+    float cu3 = cu2 * cu;
+    float cu4 = cu2 * cu2;
+    float usq2 = usq * usq;
+    float val = 1.0f + 3.0f * cu + 4.5f * cu2 - 1.5f * usq;
+    val += 4.5f * cu3 - 4.5f * cu * usq;
+    val += 3.375f * cu4 - 6.75f * cu2 * usq + 1.125f * usq2;
+    return rho * w * val;
+}
+__device__ __forceinline__ float feq_auto(float rho, float ux, float uy, float uz, float w, float cx, float cy,
+                                          float cz, int comp) {
+    return comp == 1 ? feq4(rho, ux, uy, uz, w, cx, cy, cz) : feq2(rho, ux, uy, uz, w, cx, cy, cz);
+}
+__device__ __forceinline__ float geq(float T, float ux, float uy, float uz, float w, float cx, float cy, float cz) {
+    float cu = cx * ux + cy * uy + cz * uz;
+    return w * T * (1.0f + 3.0f * cu);
+}
+
+// gpu_hash / gradient_noise (physics_utils.jl:5-16): Int32 arithmetic wraps.
+__device__ __forceinline__ float gradient_noise(int gx, int gy, int gz, int seed) {
+    uint32_t c = (uint32_t)gx * 374761393u + (uint32_t)gy * 668265263u + (uint32_t)gz * 1274126177u + (uint32_t)seed;
+    uint32_t h = c;
+    h = (h ^ (h >> 16)) * 0x85ebca6bu;
+    h = (h ^ (h >> 13)) * 0xc2b2ae35u;
+    h = h ^ (h >> 16);
+    return ((float)(h & 0xFFFFu) / 32768.0f) - 1.0f;
+}
+
+__device__ __forceinline__ float wk_rt(int k) {
+    int d2 = CX(k) * CX(k) + CY(k) * CY(k) + CZ(k) * CZ(k);
+    return d2 == 0 ? 8.0f / 27.0f : d2 == 1 ? 2.0f / 27.0f : d2 == 2 ? 1.0f / 54.0f : 1.0f / 216.0f;
+}
+
+// interpolate_with_rescaling (physics_interpolation.jl:4-127) and its thermal
+// twin (:135-248).  Parent cell P sits at coordinate P; lower corner clamped
+// to >= 1 after the weights; missing corners take corner 000's tuple.
+__device__ __noinline__ float interp_rescale(const float* __restrict__ a_new, const float* __restrict__ s_new,
+                                             const float* __restrict__ u_new, const float* __restrict__ a_old,
+                                             const float* __restrict__ s_old, const float* __restrict__ u_old,
+                                             const int32_t* __restrict__ pgrid, int pdx, int pdy, int pdz,
+                                             int fgx, int fgy, int fgz, int k, float tau_fine, float tau_coarse,
+                                             float tw, int temporal, int comp, int thermal) {
+    float w_k = wk_rt(k), cx = (float)CX(k), cy = (float)CY(k), cz = (float)CZ(k);
+    float pxc = ((float)fgx - 0.5f) * 0.5f;
+    float pyc = ((float)fgy - 0.5f) * 0.5f;
+    float pzc = ((float)fgz - 0.5f) * 0.5f;
+    int px0 = (int)floorf(pxc), py0 = (int)floorf(pyc), pz0 = (int)floorf(pzc);
+    int px1 = px0 + 1, py1 = py0 + 1, pz1 = pz0 + 1;
+    float wx = pxc - (float)px0, wy = pyc - (float)py0, wz = pzc - (float)pz0;
+    px0 = max(1, px0);
+    py0 = max(1, py0);
+    pz0 = max(1, pz0);
+    float v[8][5];
+    bool use_old = temporal == 1 && tw < 0.99f;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        int pgx = (i & 1) ? px1 : px0;
+        int pgy = (i & 2) ? py1 : py0;
+        int pgz = (i & 4) ? pz1 : pz0;
+        int pbx = (pgx - 1) / BS + 1, pby = (pgy - 1) / BS + 1, pbz = (pgz - 1) / BS + 1;
+        bool ok = false;
+        if (pbx >= 1 && pbx <= pdx && pby >= 1 && pby <= pdy && pbz >= 1 && pbz <= pdz) {
+            int pb = pgrid[(pbx - 1) + pdx * ((pby - 1) + pdy * (pbz - 1))];
+            if (pb >= 0) {
+                int c = ((pgx - 1) % BS) + BS * ((pgy - 1) % BS) + BS * BS * ((pgz - 1) % BS);
+                size_t i4 = (size_t)pb * NC + c;
+                size_t iv = (size_t)pb * 3 * NC + c;
+                float fn = a_new[fidx(pb, k, c)], rn = s_new[i4];
+                float uxn = u_new[iv], uyn = u_new[iv + NC], uzn = u_new[iv + 2 * NC];
+                if (use_old) {
+                    float fo = a_old[fidx(pb, k, c)], ro = s_old[i4];
+                    float uxo = u_old[iv], uyo = u_old[iv + NC], uzo = u_old[iv + 2 * NC];
+                    v[i][0] = fo * (1.0f - tw) + fn * tw;
+                    v[i][1] = ro * (1.0f - tw) + rn * tw;
+                    v[i][2] = uxo * (1.0f - tw) + uxn * tw;
+                    v[i][3] = uyo * (1.0f - tw) + uyn * tw;
+                    v[i][4] = uzo * (1.0f - tw) + uzn * tw;
+                } else {
+                    v[i][0] = fn; v[i][1] = rn; v[i][2] = uxn; v[i][3] = uyn; v[i][4] = uzn;
+                }
+                ok = true;
+            }
+        }
+        if (!ok) {
+            if (i == 0) {
+                v[0][0] = w_k; v[0][1] = 1.0f; v[0][2] = 0.0f; v[0][3] = 0.0f; v[0][4] = 0.0f;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 5; j++) v[i][j] = v[0][j];
+            }
+        }
+    }
+    float r[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        float c00 = v[0][j] * (1.0f - wx) + v[1][j] * wx;
+        float c01 = v[4][j] * (1.0f - wx) + v[5][j] * wx;
+        float c10 = v[2][j] * (1.0f - wx) + v[3][j] * wx;
+        float c11 = v[6][j] * (1.0f - wx) + v[7][j] * wx;
+        float c0 = c00 * (1.0f - wy) + c10 * wy;
+        float c1 = c01 * (1.0f - wy) + c11 * wy;
+        r[j] = c0 * (1.0f - wz) + c1 * wz;
+    }
+    float eq = thermal ? geq(r[1], r[2], r[3], r[4], w_k, cx, cy, cz)
+                       : feq_auto(r[1], r[2], r[3], r[4], w_k, cx, cy, cz, comp);
+    float neq = r[0] - eq;
+    float tc = tau_coarse - 0.5f;
+    float tf = tau_fine - 0.5f;
+    float scale = 1.0f;
+    if (tc > 1.0e-6f) {
+        scale = tf / tc;
+        scale = scale > 100.0f ? 100.0f : (scale < 0.01f ? 0.01f : scale);
+    }
+    return eq + neq * scale;
+}
+
+// Missing-neighbour cascade of stream_collide_cumulant! (kernel_cumulant.jl:117-159).
+__device__ __noinline__ float bc_value(const CollideArgs& a, int k, int b, int c, int sgx, int sgy, int sgz, int gy,
+                                       int gz) {
+    float cx = (float)CX(k), cy = (float)CY(k), cz = (float)CZ(k), w = wk_rt(k);
+    if (sgx < 1) {
+        float noise = a.inlet_turb > 0.0f ? gradient_noise(gy, gz, a.seed, 1234) * a.inlet_turb * a.u_inlet : 0.0f;
+        float u_inst = a.u_inlet + noise;
+        return feq_auto(1.0f, u_inst, 0.0f, 0.0f, w, cx, cy, cz, a.comp);
+    }
+    if (sgx > a.nxg) return feq_auto(1.0f, a.u_inlet, 0.0f, 0.0f, w, cx, cy, cz, a.comp);
+    if (sgy < 1 || sgy > a.nyg) return a.f_in[fidx(b, MIRY(k), c)];
+    if (sgz < 1 || sgz > a.nzg) return a.f_in[fidx(b, MIRZ(k), c)];
+    if (!a.is_l1)
+        return interp_rescale(a.pf_new, a.prho_new, a.pvel_new, a.pf_old, a.prho_old, a.pvel_old, a.pgrid, a.pdx,
+                              a.pdy, a.pdz, sgx, sgy, sgz, k, a.tau, a.tau_parent, a.tw, a.temporal, a.comp, 0);
+    return w;
+}
+
+// get_velocity_neighbor (physics_utils.jl:85-110): own-cell value when the
+// neighbour block is missing.
+__device__ __forceinline__ void vel_nb(const float* __restrict__ vin, const int* snbr, int b, int x, int y, int z,
+                                       int dx, int dy, int dz, float& u, float& v, float& w) {
+    int nx = x + dx, ny = y + dy, nz = z + dz;
+    int ox = nx < 0 ? -1 : (nx > 7 ? 1 : 0);
+    int oy = ny < 0 ? -1 : (ny > 7 ? 1 : 0);
+    int oz = nz < 0 ? -1 : (nz > 7 ? 1 : 0);
+    int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+    int blk, c;
+    if (nbb >= 0) {
+        blk = nbb;
+        c = (nx & 7) + 8 * (ny & 7) + 64 * (nz & 7);
+    } else {
+        blk = b;
+        c = x + 8 * y + 64 * z;
+    }
+    size_t iv = (size_t)blk * 3 * NC + c;
+    u = vin[iv];
+    v = vin[iv + NC];
+    w = vin[iv + 2 * NC];
+}
+
+__device__ __forceinline__ void chim_fwd(float& fm, float& f0, float& fp, float u) {
+    float m0 = fm + f0 + fp;
+    float m1 = fp - fm;
+    float m2 = fm + fp;
+    fm = m0;
+    f0 = m1 - u * m0;
+    fp = m2 - 2.0f * u * m1 + u * u * m0;
+}
+__device__ __forceinline__ void chim_bwd(float& k0, float& k1, float& k2, float u) {
+    float m0 = k0;
+    float m1 = k1 + u * k0;
+    float m2 = k2 + 2.0f * u * k1 + u * u * k0;
+    float fp = (m2 + m1) * 0.5f;
+    float fm = (m2 - m1) * 0.5f;
+    float f0 = m0 - m2;
+    k0 = fm;
+    k1 = f0;
+    k2 = fp;
+}
+
+// stream_collide_cumulant! (src/collision/kernel_cumulant.jl:29-627), with the
+// A/B buffers chosen by the caller and no f_post_collision copy (Bouzidi
+// reads f_out before it corrects it, see bouzidi_gather below).
+__global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
+    __shared__ int snbr[27];
+    const int b = blockIdx.x >> 1;
+    const int c = ((blockIdx.x & 1) << 8) | threadIdx.x;
+    if (threadIdx.x < 27) snbr[threadIdx.x] = a.nbr[(size_t)b * 27 + threadIdx.x];
+    __syncthreads();
+    const int x = c & 7, y = (c >> 3) & 7, z = c >> 6;
+    const int gx = a.borg[b * 3 + 0] + x + 1;
+    const int gy = a.borg[b * 3 + 1] + y + 1;
+    const int gz = a.borg[b * 3 + 2] + z + 1;
+    const size_t i4 = (size_t)b * NC + c;
+    const bool is_obs = a.obst[i4] != 0;
+
+    // STEP 1: pull (kernel_cumulant.jl:86-168)
+    float A[27];
+    float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+        constexpr int dummy = 0;
+        (void)dummy;
+        const int cx = CX(k), cy = CY(k), cz = CZ(k);
+        int sx = x - cx, sy = y - cy, sz = z - cz;
+        int ox = cx == 0 ? 0 : (sx < 0 ? -1 : (sx > 7 ? 1 : 0));
+        int oy = cy == 0 ? 0 : (sy < 0 ? -1 : (sy > 7 ? 1 : 0));
+        int oz = cz == 0 ? 0 : (sz < 0 ? -1 : (sz > 7 ? 1 : 0));
+        int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+        float val;
+        if (nbb >= 0) {
+            val = a.f_in[fidx(nbb, k, (sx & 7) + 8 * (sy & 7) + 64 * (sz & 7))];
+        } else {
+            val = bc_value(a, k, b, c, gx - cx, gy - cy, gz - cz, gy, gz);
+        }
+        A[k] = val;
+        rho += val;
+        if (cx == 1) jx += val;
+        if (cx == -1) jx += -val;
+        if (cy == 1) jy += val;
+        if (cy == -1) jy += -val;
+        if (cz == 1) jz += val;
+        if (cz == -1) jz += -val;
+    }
+
+    float* __restrict__ fo = a.f_out;
+    if (is_obs) {
+        // simple bounce-back of the streamed values (kernel_cumulant.jl:173-185)
+        const size_t iv = (size_t)b * 3 * NC + c;
+        a.vel_out[iv] = 0.0f;
+        a.vel_out[iv + NC] = 0.0f;
+        a.vel_out[iv + 2 * NC] = 0.0f;
+        a.rho_out[i4] = 1.0f;
+#pragma unroll
+        for (int k = 0; k < 27; k++) fo[fidx(b, k, c)] = A[OPP(k)];
+        return;
+    }
+
+    // STEP 2: macroscopic + sponge + wall model (kernel_cumulant.jl:190-259)
+    rho = fmaxf(rho, 0.01f);
+    const float inv_rho = 1.0f / rho;
+    float ux = jx * inv_rho, uy = jy * inv_rho, uz = jz * inv_rho;
+    const float sp = a.sponge[i4];
+    if (sp > 0.0f) {
+        rho = rho * (1.0f - sp) + 1.0f * sp;
+        ux = ux * (1.0f - sp) + a.u_inlet * sp;
+        uy = uy * (1.0f - sp);
+        uz = uz * (1.0f - sp);
+        if (a.sponge_blend == 1) {
+#pragma unroll
+            for (int k = 0; k < 27; k++) {
+                float ft = feq_auto(1.0f, a.u_inlet, 0.0f, 0.0f, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k),
+                                    a.comp);
+                A[k] = A[k] * (1.0f - sp) + ft * sp;
+            }
+        }
+    }
+    float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
+    if (a.wall_model == 1) {
+        const float d = a.wall_dist[i4];
+        if (d > 0.0f && d < 10.0f) {
+            float umag = sqrtf(ux * ux + uy * uy + uz * uz);
+            float nu_visc = (a.tau - 0.5f) / 3.0f;
+            if (umag > 1.0e-6f && nu_visc > 1.0e-10f) {
+                float u_tau = umag * powf(nu_visc / (d * umag + 1.0e-10f), 1.0f / 7.0f) *
+                              powf(2.0f * 8.3f, -1.0f / 7.0f);
+                u_tau = fmaxf(u_tau, 1.0e-6f);
+                float y_p = u_tau * d / nu_visc;
+                if (y_p > 11.81f) {
+                    float upl = (1.0f / 0.41f) * logf(y_p) + 5.2f;
+                    if (upl > 0.1f) {
+                        u_tau = u_tau * ((umag / u_tau) / upl);
+                        u_tau = fmaxf(u_tau, 1.0e-6f);
+                    }
+                }
+                float tau_wall = rho * u_tau * u_tau;
+                float tau_res = rho * nu_visc * (umag / d);
+                if (tau_wall > tau_res) {
+                    float fm = (tau_wall - tau_res) / d;
+                    Fx = -fm * ux / umag;
+                    Fy = -fm * uy / umag;
+                    Fz = -fm * uz / umag;
+                }
+            }
+        }
+    }
+    const float uxe = ux + 0.5f * Fx * inv_rho;
+    const float uye = uy + 0.5f * Fy * inv_rho;
+    const float uze = uz + 0.5f * Fz * inv_rho;
+    {
+        const size_t iv = (size_t)b * 3 * NC + c;
+        a.vel_out[iv] = ux;
+        a.vel_out[iv + NC] = uy;
+        a.vel_out[iv + 2 * NC] = uz;
+        a.rho_out[i4] = rho;
+    }
+
+    // STEP 3: WALE on the previous velocity (kernel_cumulant.jl:264-310)
+    float om1;
+    {
+        float uE, vE, wE, uW, vW, wW, uN, vN, wN, uS, vS, wS, uT, vT, wT, uB, vB, wB;
+        vel_nb(a.vel_in, snbr, b, x, y, z, 1, 0, 0, uE, vE, wE);
+        vel_nb(a.vel_in, snbr, b, x, y, z, -1, 0, 0, uW, vW, wW);
+        vel_nb(a.vel_in, snbr, b, x, y, z, 0, 1, 0, uN, vN, wN);
+        vel_nb(a.vel_in, snbr, b, x, y, z, 0, -1, 0, uS, vS, wS);
+        vel_nb(a.vel_in, snbr, b, x, y, z, 0, 0, 1, uT, vT, wT);
+        vel_nb(a.vel_in, snbr, b, x, y, z, 0, 0, -1, uB, vB, wB);
+        float g11 = 0.5f * (uE - uW), g12 = 0.5f * (uN - uS), g13 = 0.5f * (uT - uB);
+        float g21 = 0.5f * (vE - vW), g22 = 0.5f * (vN - vS), g23 = 0.5f * (vT - vB);
+        float g31 = 0.5f * (wE - wW), g32 = 0.5f * (wN - wS), g33 = 0.5f * (wT - wB);
+        float q11 = g11 * g11 + g12 * g21 + g13 * g31;
+        float q12 = g11 * g12 + g12 * g22 + g13 * g32;
+        float q13 = g11 * g13 + g12 * g23 + g13 * g33;
+        float q21 = g21 * g11 + g22 * g21 + g23 * g31;
+        float q22 = g21 * g12 + g22 * g22 + g23 * g32;
+        float q23 = g21 * g13 + g22 * g23 + g23 * g33;
+        float q31 = g31 * g11 + g32 * g21 + g33 * g31;
+        float q32 = g31 * g12 + g32 * g22 + g33 * g32;
+        float q33 = g31 * g13 + g32 * g23 + g33 * g33;
+        float tr = q11 + q22 + q33;
+        float trt = tr / 3.0f;
+        float Sd11 = q11 - trt, Sd22 = q22 - trt, Sd33 = q33 - trt;
+        float Sd12 = 0.5f * (q12 + q21), Sd13 = 0.5f * (q13 + q31), Sd23 = 0.5f * (q23 + q32);
+        float S12 = 0.5f * (g12 + g21), S13 = 0.5f * (g13 + g31), S23 = 0.5f * (g23 + g32);
+        float OP1 = Sd11 * Sd11 + Sd22 * Sd22 + Sd33 * Sd33 + 2.0f * (Sd12 * Sd12 + Sd13 * Sd13 + Sd23 * Sd23);
+        float OP2 = g11 * g11 + g22 * g22 + g33 * g33 + 2.0f * (S12 * S12 + S13 * S13 + S23 * S23);
+        float nu_eddy = 0.0f;
+        if (OP1 > 1.0e-12f) {
+            float OP1_32 = OP1 * sqrtf(OP1);
+            float OP2_52 = OP2 * OP2 * sqrtf(fmaxf(OP2, 1.0e-12f));
+            float den = OP2_52 + OP1 * sqrtf(sqrtf(fmaxf(OP1, 1.0e-12f)));
+            if (den > 1.0e-12f) nu_eddy = (a.c_wale * a.c_wale) * OP1_32 / den;
+        }
+        nu_eddy = fmaxf(nu_eddy, a.nu_sgs_bg);
+        float nu_mol = (a.tau - 0.5f) / 3.0f;
+        float nu_eff = nu_mol + nu_eddy;
+        float tau_eff = 0.5f + 3.0f * nu_eff;
+        om1 = 1.0f / fmaxf(tau_eff, 0.500001f);
+    }
+    const float omb = a.omega_bulk, om3 = a.omega3, om4 = a.omega4;
+    float om4d = om4;
+    if (a.adaptive == 1) {
+        float t = om1 - 2.0f;
+        float den = a.lambda * om1 + 1.0e-10f;
+        float Av = -(t * t) / den;
+        om4d = Av > 2.0f ? 2.0f : (Av < 0.01f ? 0.01f : Av);
+    }
+
+    // STEP 4: chimera forward (kernel_cumulant.jl:330-371)
+#pragma unroll
+    for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+        for (int iy = 0; iy < 3; iy++) {
+            const int bb = 3 * iy + 9 * iz;
+            chim_fwd(A[bb], A[bb + 1], A[bb + 2], uxe);
+        }
+#pragma unroll
+    for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+        for (int ma = 0; ma < 3; ma++) {
+            const int i1 = ma + 9 * iz;
+            chim_fwd(A[i1], A[i1 + 3], A[i1 + 6], uye);
+        }
+#pragma unroll
+    for (int mb = 0; mb < 3; mb++)
+#pragma unroll
+        for (int ma = 0; ma < 3; ma++) {
+            const int i1 = ma + 3 * mb;
+            chim_fwd(A[i1], A[i1 + 9], A[i1 + 18], uze);
+        }
+
+    // STEP 5-8: cumulants, relaxation, reconstruction (kernel_cumulant.jl:376-519)
+    {
+        const float k000 = A[0];
+        const float ir = 1.0f / fmaxf(k000, 0.01f);
+        const float c200 = A[2] * ir, c020 = A[6] * ir, c002 = A[18] * ir;
+        const float c110 = A[4] * ir, c101 = A[10] * ir, c011 = A[12] * ir;
+        const float c210 = A[5] * ir, c120 = A[7] * ir, c201 = A[11] * ir;
+        const float c021 = A[15] * ir, c102 = A[19] * ir, c012 = A[21] * ir;
+        const float c111 = A[13] * ir;
+        const float c220n = A[8] * ir, c202n = A[20] * ir, c022n = A[24] * ir;
+        const float c211n = A[14] * ir, c121n = A[16] * ir, c112n = A[22] * ir;
+        const float C220 = c220n - c200 * c020 - 2.0f * c110 * c110;
+        const float C202 = c202n - c200 * c002 - 2.0f * c101 * c101;
+        const float C022 = c022n - c020 * c002 - 2.0f * c011 * c011;
+        const float C211 = c211n - c200 * c011 - 2.0f * c110 * c101;
+        const float C121 = c121n - c020 * c101 - 2.0f * c110 * c011;
+        const float C112 = c112n - c002 * c110 - 2.0f * c101 * c011;
+        const float cs2 = 0.333333333f;
+        const float Dt = c200 + c020 + c002;
+        const float Dteq = 3.0f * cs2;
+        const float Dts = (1.0f - omb) * Dt + omb * Dteq;
+        const float D1 = c200 - c020, D2 = c200 - c002;
+        const float D1s = (1.0f - om1) * D1, D2s = (1.0f - om1) * D2;
+        const float c200s = (Dts + D1s + D2s) / 3.0f;
+        const float c020s = (Dts - 2.0f * D1s + D2s) / 3.0f;
+        const float c002s = (Dts + D1s - 2.0f * D2s) / 3.0f;
+        float c110s = (1.0f - om1) * c110, c101s = (1.0f - om1) * c101, c011s = (1.0f - om1) * c011;
+        if (a.limiter == 1) {
+            float b110 = sqrtf(fmaxf(c200s * c020s, 0.0f));
+            float b101 = sqrtf(fmaxf(c200s * c002s, 0.0f));
+            float b011 = sqrtf(fmaxf(c020s * c002s, 0.0f));
+            c110s = c110s > b110 ? b110 : (c110s < -b110 ? -b110 : c110s);
+            c101s = c101s > b101 ? b101 : (c101s < -b101 ? -b101 : c101s);
+            c011s = c011s > b011 ? b011 : (c011s < -b011 ? -b011 : c011s);
+        }
+        const float c210s = (1.0f - om3) * c210, c120s = (1.0f - om3) * c120, c201s = (1.0f - om3) * c201;
+        const float c021s = (1.0f - om3) * c021, c102s = (1.0f - om3) * c102, c012s = (1.0f - om3) * c012;
+        const float c111s = (1.0f - om3) * c111;
+        const float C220s = (1.0f - om4d) * C220, C202s = (1.0f - om4d) * C202, C022s = (1.0f - om4d) * C022;
+        const float C211s = (1.0f - om4) * C211, C121s = (1.0f - om4) * C121, C112s = (1.0f - om4) * C112;
+        const float c220s = C220s + c200s * c020s + 2.0f * c110s * c110s;
+        const float c202s = C202s + c200s * c002s + 2.0f * c101s * c101s;
+        const float c022s = C022s + c020s * c002s + 2.0f * c011s * c011s;
+        const float c211s = C211s + c200s * c011s + 2.0f * c110s * c101s;
+        const float c121s = C121s + c020s * c101s + 2.0f * c110s * c011s;
+        const float c112s = C112s + c002s * c110s + 2.0f * c101s * c011s;
+        const float c221s = c200s * c021s + c020s * c201s + 4.0f * c110s * c111s + 2.0f * c101s * c120s +
+                            2.0f * c011s * c210s;
+        const float c212s = c200s * c012s + c002s * c210s + 2.0f * c110s * c102s + 4.0f * c101s * c111s +
+                            2.0f * c011s * c201s;
+        const float c122s = c020s * c102s + c002s * c120s + 2.0f * c110s * c012s + 4.0f * c011s * c111s +
+                            2.0f * c101s * c021s;
+        float c222s = c200s * C022s + c020s * C202s + c002s * C220s +
+                      4.0f * (c110s * C112s + c101s * C121s + c011s * C211s);
+        c222s += 2.0f * (c210s * c012s + c201s * c021s + c120s * c102s) + 4.0f * c111s * c111s;
+        c222s += c200s * c020s * c002s +
+                 2.0f * (c200s * c011s * c011s + c020s * c101s * c101s + c002s * c110s * c110s) +
+                 8.0f * c110s * c101s * c011s;
+        const float r = k000;
+        A[0] = r;
+        A[1] = 0.0f; A[3] = 0.0f; A[9] = 0.0f;
+        A[2] = r * c200s; A[6] = r * c020s; A[18] = r * c002s;
+        A[4] = r * c110s; A[10] = r * c101s; A[12] = r * c011s;
+        A[5] = r * c210s; A[7] = r * c120s; A[11] = r * c201s;
+        A[15] = r * c021s; A[19] = r * c102s; A[21] = r * c012s;
+        A[13] = r * c111s;
+        A[8] = r * c220s; A[20] = r * c202s; A[24] = r * c022s;
+        A[14] = r * c211s; A[16] = r * c121s; A[22] = r * c112s;
+        A[17] = r * c221s; A[23] = r * c212s; A[25] = r * c122s;
+        A[26] = r * c222s;
+    }
+
+    // STEP 9: chimera backward (kernel_cumulant.jl:525-559)
+#pragma unroll
+    for (int mb = 0; mb < 3; mb++)
+#pragma unroll
+        for (int ma = 0; ma < 3; ma++) {
+            const int i1 = ma + 3 * mb;
+            chim_bwd(A[i1], A[i1 + 9], A[i1 + 18], uze);
+        }
+#pragma unroll
+    for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+        for (int ma = 0; ma < 3; ma++) {
+            const int i1 = ma + 9 * iz;
+            chim_bwd(A[i1], A[i1 + 3], A[i1 + 6], uye);
+        }
+#pragma unroll
+    for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+        for (int iy = 0; iy < 3; iy++) {
+            const int bb = 3 * iy + 9 * iz;
+            chim_bwd(A[bb], A[bb + 1], A[bb + 2], uxe);
+        }
+
+    // positivity limiter (kernel_cumulant.jl:564-599)
+    if (a.limiter == 2) {
+        float fmin_ = A[0];
+#pragma unroll
+        for (int k = 1; k < 27; k++) fmin_ = fminf(fmin_, A[k]);
+        if (fmin_ < 0.0f) {
+            float alpha = 1.0f;
+#pragma unroll
+            for (int k = 0; k < 27; k++) {
+                float fe = feq_auto(rho, uxe, uye, uze, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k), a.comp);
+                float fn = A[k] - fe;
+                if (fn < 0.0f && fe > 1.0e-15f) {
+                    float ak = fe / (fe - fn + 1.0e-15f);
+                    alpha = fminf(alpha, ak);
+                }
+            }
+            alpha = fmaxf(alpha, 0.0f);
+#pragma unroll
+            for (int k = 0; k < 27; k++) {
+                float fe = feq_auto(rho, uxe, uye, uze, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k), a.comp);
+                A[k] = fe + alpha * (A[k] - fe);
+            }
+        }
+    }
+
+    // STEP 10: Guo forcing + store (kernel_cumulant.jl:604-623).  With F = 0
+    // the forcing term is a signed zero and leaves every population unchanged.
+    if (Fx != 0.0f || Fy != 0.0f || Fz != 0.0f) {
+        const float pre = 1.0f - 0.5f * om1;
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+            const float cxf = (float)CX(k), cyf = (float)CY(k), czf = (float)CZ(k);
+            float cu = cxf * uxe + cyf * uye + czf * uze;
+            float ft = WK(k) * 3.0f *
+                       ((cxf - ux + 3.0f * cu * cxf) * Fx + (cyf - uy + 3.0f * cu * cyf) * Fy +
+                        (czf - uz + 3.0f * cu * czf) * Fz);
+            fo[fidx(b, k, c)] = A[k] + pre * ft;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 27; k++) fo[fidx(b, k, c)] = A[k];
+    }
+}
+
+hipError_t collide(const CollideArgs& a, hipStream_t s) {
+    if (a.nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(collide_kernel, dim3((unsigned)a.nb * 2), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Thermal DDF (src/collision/kernel_thermal.jl:14-202)
+__device__ __noinline__ float thermal_bc(const ThermalArgs& a, int k, int b, int c, int sgx, int sgy, int sgz) {
+    float cx = (float)CX(k), cy = (float)CY(k), cz = (float)CZ(k), w = wk_rt(k);
+    if (sgx < 1) return geq(a.T_inlet, a.u_inlet, 0.0f, 0.0f, w, cx, cy, cz);
+    if (sgx > a.nxg) return geq(a.T_inlet, a.u_inlet, 0.0f, 0.0f, w, cx, cy, cz);
+    if (sgy < 1 || sgy > a.nyg) return a.g_in[fidx(b, MIRY(k), c)];
+    if (sgz < 1 || sgz > a.nzg) return a.g_in[fidx(b, MIRZ(k), c)];
+    if (!a.is_l1)
+        return interp_rescale(a.pg_new, a.pT_new, a.pvel_new, a.pg_old, a.pT_old, a.pvel_old, a.pgrid, a.pdx, a.pdy,
+                              a.pdz, sgx, sgy, sgz, k, a.tau_g, a.tau_g_parent, a.tw, a.temporal, 0, 1);
+    return w * a.T_inlet;
+}
+
+__global__ void __launch_bounds__(256) thermal_kernel(const ThermalArgs a) {
+    __shared__ int snbr[27];
+    const int b = blockIdx.x >> 1;
+    const int c = ((blockIdx.x & 1) << 8) | threadIdx.x;
+    if (threadIdx.x < 27) snbr[threadIdx.x] = a.nbr[(size_t)b * 27 + threadIdx.x];
+    __syncthreads();
+    const int x = c & 7, y = (c >> 3) & 7, z = c >> 6;
+    const int gx = a.borg[b * 3 + 0] + x + 1;
+    const int gy = a.borg[b * 3 + 1] + y + 1;
+    const int gz = a.borg[b * 3 + 2] + z + 1;
+    const size_t i4 = (size_t)b * NC + c;
+    const size_t iv = (size_t)b * 3 * NC + c;
+    const bool is_obs = a.obst[i4] != 0;
+    float ux = a.vel[iv], uy = a.vel[iv + NC], uz = a.vel[iv + 2 * NC];
+    float G[27];
+    float Tsum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+        const int cx = CX(k), cy = CY(k), cz = CZ(k);
+        int sx = x - cx, sy = y - cy, sz = z - cz;
+        int ox = cx == 0 ? 0 : (sx < 0 ? -1 : (sx > 7 ? 1 : 0));
+        int oy = cy == 0 ? 0 : (sy < 0 ? -1 : (sy > 7 ? 1 : 0));
+        int oz = cz == 0 ? 0 : (sz < 0 ? -1 : (sz > 7 ? 1 : 0));
+        int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+        float val;
+        if (nbb >= 0) val = a.g_in[fidx(nbb, k, (sx & 7) + 8 * (sy & 7) + 64 * (sz & 7))];
+        else val = thermal_bc(a, k, b, c, gx - cx, gy - cy, gz - cz);
+        Tsum += val;
+        G[k] = val;
+    }
+    if (is_obs) {
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+            float gopp = a.g_in[fidx(b, OPP(k), c)];  // local, unstreamed (kernel_thermal.jl:140)
+            float go;
+            if (a.adiabatic == 1) {
+                go = gopp;
+            } else {
+                float gw = geq(a.T_wall, 0.0f, 0.0f, 0.0f, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k));
+                go = 2.0f * gw - gopp;
+            }
+            a.g_out[fidx(b, k, c)] = go;
+        }
+        a.T_out[i4] = a.adiabatic == 1 ? Tsum : a.T_wall;
+        return;
+    }
+    float Tl = fmaxf(Tsum, 0.01f);
+    const float sp = a.sponge[i4];
+    if (sp > 0.0f) {
+        Tl = Tl * (1.0f - sp) + a.T_inlet * sp;
+        ux = ux * (1.0f - sp) + a.u_inlet * sp;
+        uy = uy * (1.0f - sp);
+        uz = uz * (1.0f - sp);
+    }
+    a.T_out[i4] = Tl;
+    const float omg = 1.0f / fmaxf(a.tau_g, 0.500001f);
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+        float gs = G[k];
+        if (sp > 0.0f && a.sponge_blend == 1) {
+            float gt = geq(a.T_inlet, a.u_inlet, 0.0f, 0.0f, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k));
+            gs = gs * (1.0f - sp) + gt * sp;
+        }
+        float ge = geq(Tl, ux, uy, uz, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k));
+        a.g_out[fidx(b, k, c)] = gs - omg * (gs - ge);
+    }
+}
+
+hipError_t thermal(const ThermalArgs& a, hipStream_t s) {
+    if (a.nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(thermal_kernel, dim3((unsigned)a.nb * 2), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Bouzidi (src/geometry/bouzidi_kernel.jl:13-92) as two sparse passes over
+// the link list: gather every corrected value from the uncorrected
+// post-collision populations, then scatter.  Each link's destination slot is
+// distinct, so the result equals the reference's read-from-f_post_collision
+// form without its full-size copy.  __fmul_rn/__fadd_rn keep the two
+// products and the sum rounded separately in both builds.
+__global__ void bouzidi_gather(const BouzidiLinks l, const float* __restrict__ f) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= l.n) return;
+    l.tmp[i] = __fadd_rn(__fmul_rn(l.a[i], f[l.s1[i]]), __fmul_rn(l.b[i], f[l.s2[i]]));
+}
+__global__ void bouzidi_scatter(const BouzidiLinks l, float* __restrict__ f) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= l.n) return;
+    f[l.dst[i]] = l.tmp[i];
+}
+
+hipError_t bouzidi(const BouzidiLinks& l, float* f, hipStream_t s) {
+    if (l.n <= 0) return hipSuccess;
+    unsigned g = (unsigned)((l.n + 255) / 256);
+    hipLaunchKernelGGL(bouzidi_gather, dim3(g), dim3(256), 0, s, l, (const float*)f);
+    hipLaunchKernelGGL(bouzidi_scatter, dim3(g), dim3(256), 0, s, l, f);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// init_eq! (main.jl:113-128): f = f_temp = w, rho = 1, vel = 0 on both halves
+// of every ping-pong pair (the old-state aliases included).
+__global__ void init_kernel(float* f0, float* f1, float* rho0, float* rho1, float* v0, float* v1) {
+    const int b = blockIdx.x >> 1;
+    const int c = ((blockIdx.x & 1) << 8) | threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+        f0[fidx(b, k, c)] = WK(k);
+        f1[fidx(b, k, c)] = WK(k);
+    }
+    rho0[(size_t)b * NC + c] = 1.0f;
+    rho1[(size_t)b * NC + c] = 1.0f;
+    size_t iv = (size_t)b * 3 * NC + c;
+    for (int d = 0; d < 3; d++) {
+        v0[iv + d * NC] = 0.0f;
+        v1[iv + d * NC] = 0.0f;
+    }
+}
+hipError_t init_level(float* f0, float* f1, float* rho0, float* rho1, float* v0, float* v1, int32_t nb,
+                      hipStream_t s) {
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(init_kernel, dim3((unsigned)nb * 2), dim3(256), 0, s, f0, f1, rho0, rho1, v0, v1);
+    return hipGetLastError();
+}
+
+// init_thermal_eq! (main.jl:131-150)
+__global__ void init_thermal_kernel(float* g0, float* g1, float* T0, float* T1, float Tinit) {
+    const int b = blockIdx.x >> 1;
+    const int c = ((blockIdx.x & 1) << 8) | threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+        float ge = WK(k) * Tinit;
+        g0[fidx(b, k, c)] = ge;
+        g1[fidx(b, k, c)] = ge;
+    }
+    T0[(size_t)b * NC + c] = Tinit;
+    T1[(size_t)b * NC + c] = Tinit;
+}
+hipError_t init_thermal(float* g0, float* g1, float* T0, float* T1, int32_t nb, float Tinit, hipStream_t s) {
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(init_thermal_kernel, dim3((unsigned)nb * 2), dim3(256), 0, s, g0, g1, T0, T1, Tinit);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Surface forces (src/forces/surface.jl:143-507, 702-762).
+__device__ __forceinline__ bool cell_at(const ForceArgs& a, int gx, int gy, int gz, int& blk, int& c) {
+    if (gx < 1 || gy < 1 || gz < 1) return false;
+    int bx = (gx - 1) / BS + 1, by = (gy - 1) / BS + 1, bz = (gz - 1) / BS + 1;
+    if (bx < 1 || bx > a.gdx || by < 1 || by > a.gdy || bz < 1 || bz > a.gdz) return false;
+    int id = a.grid[(bx - 1) + a.gdx * ((by - 1) + a.gdy * (bz - 1))];
+    if (id < 0) return false;
+    blk = id;
+    c = ((gx - 1) % BS) + BS * ((gy - 1) % BS) + BS * BS * ((gz - 1) % BS);
+    return true;
+}
+
+// trilinear_sample_flow (surface.jl:224-346): corners in the reference order.
+__device__ bool trilinear_sample(const ForceArgs& a, float px, float py, float pz, float& ro, float& uo, float& vo,
+                                 float& wo, float& wq) {
+    float inv_dx = 1.0f / a.dx;
+    float fx = px * inv_dx - 0.5f, fy = py * inv_dx - 0.5f, fz = pz * inv_dx - 0.5f;
+    int ix0 = (int)floorf(fx) + 1, iy0 = (int)floorf(fy) + 1, iz0 = (int)floorf(fz) + 1;
+    float wx = fminf(fmaxf(fx - (float)(ix0 - 1), 0.0f), 1.0f);
+    float wy = fminf(fmaxf(fy - (float)(iy0 - 1), 0.0f), 1.0f);
+    float wz = fminf(fmaxf(fz - (float)(iz0 - 1), 0.0f), 1.0f);
+    float w0x = 1.0f - wx, w1x = wx, w0y = 1.0f - wy, w1y = wy, w0z = 1.0f - wz, w1z = wz;
+    float racc = 0.0f, uacc = 0.0f, vacc = 0.0f, wacc_u = 0.0f, wacc = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        int dxi = i & 1, dyi = (i >> 1) & 1, dzi = (i >> 2) & 1;
+        float wk = (dxi ? w1x : w0x) * (dyi ? w1y : w0y) * (dzi ? w1z : w0z);
+        int blk, c;
+        if (cell_at(a, ix0 + dxi, iy0 + dyi, iz0 + dzi, blk, c)) {
+            size_t i4 = (size_t)blk * NC + c;
+            if (!a.obst[i4]) {
+                size_t iv = (size_t)blk * 3 * NC + c;
+                racc += wk * a.rho[i4];
+                uacc += wk * a.vel[iv];
+                vacc += wk * a.vel[iv + NC];
+                wacc_u += wk * a.vel[iv + 2 * NC];
+                wacc += wk;
+            }
+        }
+    }
+    if (wacc > 0.01f) {
+        float iw = 1.0f / wacc;
+        ro = racc * iw; uo = uacc * iw; vo = vacc * iw; wo = wacc_u * iw; wq = wacc;
+        return true;
+    }
+    ro = 1.0f; uo = 0.0f; vo = 0.0f; wo = 0.0f; wq = 0.0f;
+    return false;
+}
+
+// map_stresses_kernel! + integrate_forces_kernel! per triangle; the nine sums
+// are reduced in FP64 in a fixed tree order per work-group (deterministic).
+__global__ void __launch_bounds__(256) forces_kernel(const ForceArgs a) {
+    __shared__ double red[9][256];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float d9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < a.n_tri) {
+        float tx = a.cx[i] + a.off_x, ty = a.cy[i] + a.off_y, tz = a.cz[i] + a.off_z;
+        float n_x = a.nx[i], n_y = a.ny[i], n_z = a.nz[i];
+        float brho = 1.0f, bux = 0.0f, buy = 0.0f, buz = 0.0f, bwd = 1.0f, bq = 0.0f;
+        bool found = false;
+        for (int pi = 1; pi <= a.search_radius; pi++) {
+            float dc = (float)pi + 0.5f;
+            float px = tx + n_x * dc * a.dx, py = ty + n_y * dc * a.dx, pz = tz + n_z * dc * a.dx;
+            float r, u, v, w, wq;
+            bool ok = trilinear_sample(a, px, py, pz, r, u, v, w, wq);
+            if (ok && wq >= 0.5f) {
+                brho = r; bux = u; buy = v; buz = w; bwd = dc; bq = wq; found = true;
+                break;
+            } else if (ok && wq > bq) {
+                brho = r; bux = u; buy = v; buz = w; bwd = dc; bq = wq; found = true;
+            }
+        }
+        if (!found) {
+            float gxf = tx / a.dx, gyf = ty / a.dx, gzf = tz / a.dx;
+            int g_x = (int)floorf(gxf) + 1, g_y = (int)floorf(gyf) + 1, g_z = (int)floorf(gzf) + 1;
+            float best = 1e10f;
+            for (int rad = 1; rad <= a.search_radius; rad++) {
+                if (found) break;
+                for (int dz = -rad; dz <= rad; dz++)
+                    for (int dy = -rad; dy <= rad; dy++)
+                        for (int ddx = -rad; ddx <= rad; ddx++) {
+                            bool shell = (abs(ddx) == rad) || (abs(dy) == rad) || (abs(dz) == rad);
+                            if (!shell) continue;
+                            int cgx = g_x + ddx, cgy = g_y + dy, cgz = g_z + dz;
+                            int blk, c;
+                            if (!cell_at(a, cgx, cgy, cgz, blk, c)) continue;
+                            size_t i4 = (size_t)blk * NC + c;
+                            if (a.obst[i4]) continue;
+                            float ccx = ((float)cgx - 0.5f) * a.dx, ccy = ((float)cgy - 0.5f) * a.dx,
+                                  ccz = ((float)cgz - 0.5f) * a.dx;
+                            float dxp = ccx - tx, dyp = ccy - ty, dzp = ccz - tz;
+                            float dn = dxp * n_x + dyp * n_y + dzp * n_z;
+                            if (dn <= 0.0f) continue;
+                            float dsq = dxp * dxp + dyp * dyp + dzp * dzp;
+                            if (dsq < best) {
+                                size_t iv = (size_t)blk * 3 * NC + c;
+                                best = dsq;
+                                brho = a.rho[i4];
+                                bux = a.vel[iv]; buy = a.vel[iv + NC]; buz = a.vel[iv + 2 * NC];
+                                bwd = dn / a.dx;
+                                found = true;
+                            }
+                        }
+            }
+        }
+        float p = 0.0f, tx_ = 0.0f, ty_ = 0.0f, tz_ = 0.0f;
+        if (found) {
+            // compute_stress_from_cell (surface.jl:143-190)
+            float wd = fmaxf(bwd, 0.5f);
+            float pg = (brho - 1.0f) / 3.0f;
+            p = pg * a.pressure_scale;
+            float und = bux * n_x + buy * n_y + buz * n_z;
+            float utx = bux - und * n_x, uty = buy - und * n_y, utz = buz - und * n_z;
+            float utm = sqrtf(utx * utx + uty * uty + utz * utz);
+            float nu_mol = (a.tau_mol - 0.5f) / 3.0f;
+            float nu_sgs = 0.0f;
+            if (a.c_wale > 0.0f && wd > 0.01f && utm > 1.0e-10f) {
+                float S = utm / wd;
+                nu_sgs = (a.c_wale * a.c_wale) * S;
+            }
+            nu_sgs = fmaxf(nu_sgs, a.nu_sgs_bg);
+            float nu_eff = nu_mol + nu_sgs;
+            if (utm > 1.0e-10f && wd > 0.01f) {
+                float tl = brho * nu_eff * utm / wd;
+                float tp = tl * a.stress_scale;
+                tx_ = (utx / utm) * tp;
+                ty_ = (uty / utm) * tp;
+                tz_ = (utz / utm) * tp;
+            }
+        }
+        if (a.p_map) {
+            a.p_map[i] = p; a.sx_map[i] = tx_; a.sy_map[i] = ty_; a.sz_map[i] = tz_;
+        }
+        // integrate_forces_kernel! per-triangle terms (surface.jl:716-748)
+        float A = a.area[i];
+        float dFpx = -p * n_x * A, dFpy = -p * n_y * A, dFpz = -p * n_z * A;
+        float dFvx = tx_ * A, dFvy = ty_ * A, dFvz = tz_ * A;
+        float dFx = dFpx + dFvx, dFy = dFpy + dFvy, dFz = dFpz + dFvz;
+        float rx = (a.cx[i] + a.off_x) - a.ref_x, ry = (a.cy[i] + a.off_y) - a.ref_y, rz = (a.cz[i] + a.off_z) - a.ref_z;
+        d9[0] = dFpx; d9[1] = dFpy; d9[2] = dFpz;
+        d9[3] = dFvx; d9[4] = dFvy; d9[5] = dFvz;
+        d9[6] = ry * dFz - rz * dFy;
+        d9[7] = rz * dFx - rx * dFz;
+        d9[8] = rx * dFy - ry * dFx;
+    }
+#pragma unroll
+    for (int j = 0; j < 9; j++) red[j][threadIdx.x] = (double)d9[j];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+#pragma unroll
+            for (int j = 0; j < 9; j++) red[j][threadIdx.x] += red[j][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 9) a.partials[(size_t)blockIdx.x * 9 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+hipError_t forces(const ForceArgs& a, int n_groups, double* out9, hipStream_t s) {
+    (void)out9;
+    if (a.n_tri <= 0) return hipSuccess;
+    hipLaunchKernelGGL(forces_kernel, dim3((unsigned)n_groups), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 diagnostics per block: sum rho, KE, min/max rho over fluid cells,
+// non-finite populations, fluid count, sum of populations.
+__global__ void __launch_bounds__(256) stats_kernel(const float* __restrict__ f, const float* __restrict__ rho,
+                                                    const float* __restrict__ vel, const uint8_t* __restrict__ obst,
+                                                    int32_t nb, double* partials) {
+    __shared__ double red[7][256];
+    double s_rho = 0, s_ke = 0, mn = 1e300, mx = -1e300, nonfin = 0, nfl = 0, s_f = 0;
+    for (int64_t blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+        for (int c = threadIdx.x; c < NC; c += 256) {
+            size_t i4 = (size_t)blk * NC + c;
+            double cell_f = 0;
+            for (int k = 0; k < 27; k++) {
+                float v = f[fidx((int)blk, k, c)];
+                if (!isfinite(v)) nonfin += 1;
+                cell_f += (double)v;
+            }
+            s_f += cell_f;
+            if (!obst[i4]) {
+                double r = rho[i4];
+                size_t iv = (size_t)blk * 3 * NC + c;
+                double u = vel[iv], v = vel[iv + NC], w = vel[iv + 2 * NC];
+                s_rho += r;
+                s_ke += 0.5 * r * (u * u + v * v + w * w);
+                mn = r < mn ? r : mn;
+                mx = r > mx ? r : mx;
+                nfl += 1;
+            }
+        }
+    }
+    red[0][threadIdx.x] = s_rho; red[1][threadIdx.x] = s_ke; red[2][threadIdx.x] = mn; red[3][threadIdx.x] = mx;
+    red[4][threadIdx.x] = nonfin; red[5][threadIdx.x] = nfl; red[6][threadIdx.x] = s_f;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+            red[2][threadIdx.x] = fmin(red[2][threadIdx.x], red[2][threadIdx.x + s]);
+            red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + s]);
+            red[4][threadIdx.x] += red[4][threadIdx.x + s];
+            red[5][threadIdx.x] += red[5][threadIdx.x + s];
+            red[6][threadIdx.x] += red[6][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) partials[(size_t)blockIdx.x * 7 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+hipError_t stats(const float* f, const float* rho, const float* vel, const uint8_t* obst, int32_t nb,
+                 double* partials, int n_groups, double* out, hipStream_t s) {
+    (void)out;
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(stats_kernel, dim3((unsigned)n_groups), dim3(256), 0, s, f, rho, vel, obst, nb, partials);
+    return hipGetLastError();
+}
+
+}  // namespace LBK_NS
