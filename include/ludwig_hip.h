@@ -192,6 +192,14 @@ int lb_stats(lb_ctx* ctx, int level_id, double* out);
 /* Device bytes the context holds (serving admission).                    */
 int64_t lb_device_bytes(const lb_ctx* ctx);
 
+/* Kernel timing (no counterpart in the reference, which only times the host
+ * loop, main.jl:232): when enabled, HIP events are recorded on the context's
+ * stream around every stream-collide launch.  lb_profile_read synchronises,
+ * returns the summed kernel time (ms) and launch count of level_id since the
+ * last read, and resets them.                                              */
+int lb_profile_enable(lb_ctx* ctx, int enable);
+int lb_profile_read(lb_ctx* ctx, int level_id, double* ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

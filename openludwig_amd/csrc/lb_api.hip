@@ -78,6 +78,9 @@ struct Level {
     float* lb = nullptr;
     float* ltmp = nullptr;
     int cur = 0;  // buffer index written by the last step (0 before any step)
+    // kernel timing: event pairs recorded around collide launches
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t ev_used = 0;
 };
 
 struct Parent {
@@ -105,6 +108,7 @@ struct lb_ctx {
     double* fpart = nullptr;
     double* spart = nullptr;
     int n_stat_groups = 0;
+    bool profile = false;
 };
 
 namespace {
@@ -318,7 +322,20 @@ int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, floa
         } else {
             a.tau_parent = 0.5f;
         }
+        hipEvent_t e1 = nullptr;
+        if (c->profile) {
+            if (L.ev_used == L.ev.size()) {
+                hipEvent_t x, y;
+                HIPCHK(c, hipEventCreate(&x));
+                HIPCHK(c, hipEventCreate(&y));
+                L.ev.emplace_back(x, y);
+            }
+            HIPCHK(c, hipEventRecord(L.ev[L.ev_used].first, c->stream));
+            e1 = L.ev[L.ev_used].second;
+            L.ev_used++;
+        }
         HIPCHK(c, c->k->collide(a, c->stream));
+        if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
         if (L.nlinks > 0) HIPCHK(c, c->k->bouzidi(links_of(L), L.f[out], c->stream));
         if (ddf) {
             lbk::ThermalArgs t;
@@ -430,6 +447,11 @@ void lb_destroy(lb_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& L : c->levels)
+        for (auto& e : L.ev) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -750,5 +772,28 @@ int lb_stats(lb_ctx* c, int level_id, double* out) {
 }
 
 int64_t lb_device_bytes(const lb_ctx* c) { return c ? c->bytes : 0; }
+
+int lb_profile_enable(lb_ctx* c, int enable) {
+    if (!c) return LB_EINVAL;
+    c->profile = enable != 0;
+    return LB_OK;
+}
+
+int lb_profile_read(lb_ctx* c, int level_id, double* ms, int64_t* launches) {
+    if (!c || !ms || !launches || level_id < 1 || level_id > (int)c->levels.size()) return LB_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    Level& L = c->levels[(size_t)level_id - 1];
+    double tot = 0.0;
+    for (size_t i = 0; i < L.ev_used; i++) {
+        float t = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&t, L.ev[i].first, L.ev[i].second));
+        tot += (double)t;
+    }
+    *ms = tot;
+    *launches = (int64_t)L.ev_used;
+    L.ev_used = 0;
+    return LB_OK;
+}
 
 }  // extern "C"

@@ -216,6 +216,16 @@ class Solver:
         a = np.ascontiguousarray(data, dtype=np.float32)
         self._check(self.lib.lb_upload_level(self._ctx, level_id, code, N.ptr(a, C.c_float)), "lb_upload_level")
 
+    def profile(self, enable: bool = True) -> None:
+        self._check(self.lib.lb_profile_enable(self._ctx, int(enable)), "lb_profile_enable")
+
+    def profile_read(self, level_id: int) -> tuple:
+        """(summed collide kernel ms, launches) of a level since the last read."""
+        ms = C.c_double()
+        n = C.c_int64()
+        self._check(self.lib.lb_profile_read(self._ctx, level_id, C.byref(ms), C.byref(n)), "lb_profile_read")
+        return ms.value, n.value
+
     def stats(self, level_id: int = 1) -> dict:
         out = np.zeros(8)
         self._check(self.lib.lb_stats(self._ctx, level_id, N.ptr(out, C.c_double)), "lb_stats")
