@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "ludwig_hip.h"
@@ -39,6 +40,7 @@ struct Kern {
     hipError_t (*collide)(const lbk::CollideArgs&, hipStream_t);
     hipError_t (*thermal)(const lbk::ThermalArgs&, hipStream_t);
     hipError_t (*bouzidi)(const lbk::BouzidiLinks&, float*, hipStream_t);
+    hipError_t (*ghost_fill)(const lbk::GhostArgs&, hipStream_t);
     hipError_t (*init_level)(float*, float*, float*, float*, float*, float*, int32_t, hipStream_t);
     hipError_t (*init_thermal)(float*, float*, float*, float*, int32_t, float, hipStream_t);
     hipError_t (*forces)(const lbk::ForceArgs&, int, double*, hipStream_t);
@@ -46,9 +48,11 @@ struct Kern {
                         hipStream_t);
 };
 
-const Kern kStrict = {lbk_strict::collide, lbk_strict::thermal, lbk_strict::bouzidi, lbk_strict::init_level,
+const Kern kStrict = {lbk_strict::collide, lbk_strict::thermal, lbk_strict::bouzidi, lbk_strict::ghost_fill,
+                      lbk_strict::init_level,
                       lbk_strict::init_thermal, lbk_strict::forces, lbk_strict::stats};
-const Kern kFast = {lbk_fast::collide, lbk_fast::thermal, lbk_fast::bouzidi, lbk_fast::init_level,
+const Kern kFast = {lbk_fast::collide, lbk_fast::thermal, lbk_fast::bouzidi, lbk_fast::ghost_fill,
+                    lbk_fast::init_level,
                     lbk_fast::init_thermal, lbk_fast::forces, lbk_fast::stats};
 
 struct Level {
@@ -67,7 +71,12 @@ struct Level {
     float* wall_dist = nullptr;
     int32_t* nbr = nullptr;
     int32_t* borg = nullptr;
+    int32_t* order = nullptr;
     int32_t* grid = nullptr;
+    // ghost blocks (coarse-to-fine interface sources)
+    int32_t ngh = 0;
+    int64_t ngent = 0;
+    int32_t* gent = nullptr;
     int32_t pd[3] = {0, 0, 0};
     // Bouzidi link list
     int64_t nlinks = 0;
@@ -283,6 +292,7 @@ int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, floa
         a.wall_dist = L.wall_dist;
         a.nbr = L.nbr;
         a.borg = L.borg;
+        a.order = L.order;
         a.nb = L.nb;
         const int scale = 1 << (L.id - 1);
         a.nxg = c->cfg.domain_nx * scale;
@@ -322,6 +332,25 @@ int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, floa
         } else {
             a.tau_parent = 0.5f;
         }
+        if (pv && L.ngent > 0) {
+            lbk::GhostArgs gh;
+            std::memset(&gh, 0, sizeof gh);
+            gh.f = L.f[par];
+            gh.entries = L.gent;
+            gh.n = L.ngent;
+            gh.borg = L.borg;
+            gh.a_new = pv->f_new; gh.s_new = pv->rho_new; gh.u_new = pv->vel_new;
+            gh.a_old = pv->f_old; gh.s_old = pv->rho_old; gh.u_old = pv->vel_old;
+            gh.pgrid = pv->P->grid;
+            gh.pdx = pv->P->pd[0]; gh.pdy = pv->P->pd[1]; gh.pdz = pv->P->pd[2];
+            gh.tau_fine = L.tau;
+            gh.tau_coarse = pv->P->tau;
+            gh.tw = tw;
+            gh.temporal = a.temporal;
+            gh.comp = a.comp;
+            gh.thermal = 0;
+            HIPCHK(c, c->k->ghost_fill(gh, c->stream));
+        }
         hipEvent_t e1 = nullptr;
         if (c->profile) {
             if (L.ev_used == L.ev.size()) {
@@ -348,6 +377,7 @@ int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, floa
             t.sponge = L.sponge;
             t.nbr = L.nbr;
             t.borg = L.borg;
+            t.order = L.order;
             t.nb = L.nb;
             t.nxg = a.nxg;
             t.nyg = a.nyg;
@@ -376,6 +406,24 @@ int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, floa
                 t.tau_g_parent = pv->P->tau_g;
             } else {
                 t.tau_g_parent = 0.5f;
+            }
+            if (pv && L.ngent > 0) {
+                lbk::GhostArgs gh;
+                std::memset(&gh, 0, sizeof gh);
+                gh.f = L.g[par];
+                gh.entries = L.gent;
+                gh.n = L.ngent;
+                gh.borg = L.borg;
+                gh.a_new = pv->g_new; gh.s_new = pv->T_new; gh.u_new = pv->vel_new;
+                gh.a_old = pv->g_old; gh.s_old = pv->T_old; gh.u_old = pv->vel_old;
+                gh.pgrid = pv->P->grid;
+                gh.pdx = pv->P->pd[0]; gh.pdy = pv->P->pd[1]; gh.pdz = pv->P->pd[2];
+                gh.tau_fine = L.tau_g;
+                gh.tau_coarse = pv->P->tau_g;
+                gh.tw = tw;
+                gh.temporal = a.temporal;
+                gh.thermal = 1;
+                HIPCHK(c, c->k->ghost_fill(gh, c->stream));
             }
             HIPCHK(c, c->k->thermal(t, c->stream));
             if (L.nlinks > 0) HIPCHK(c, c->k->bouzidi(links_of(L), L.g[out], c->stream));
@@ -477,7 +525,66 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     L.tau = d->tau;
     L.tau_g = d->tau_g;
     L.dx = d->dx;
-    const size_t n4 = (size_t)nb * NC, n5 = n4 * Q;
+    // Host tables: 0-based neighbour rows, block origins, and ghost blocks for
+    // in-domain neighbour positions the level does not hold (levels > 1).
+    std::vector<int32_t> nbr((size_t)nb * Q), borg((size_t)nb * 3), gent;
+    const int scale = 1 << (L.id - 1);
+    const int64_t dbx = (int64_t)c->cfg.domain_nx * scale / 8, dby = (int64_t)c->cfg.domain_ny * scale / 8,
+                  dbz = (int64_t)c->cfg.domain_nz * scale / 8;
+    auto key = [](int64_t x, int64_t y, int64_t z) { return (uint64_t)(x + 1) | ((uint64_t)(y + 1) << 21) | ((uint64_t)(z + 1) << 42); };
+    std::unordered_map<uint64_t, int32_t> real, ghost;
+    std::vector<int64_t> gpos;
+    for (int32_t b = 0; b < nb; b++) real[key(d->map_x[b], d->map_y[b], d->map_z[b])] = b;
+    for (int32_t b = 0; b < nb; b++) {
+        for (int k = 0; k < Q; k++) {
+            int v = d->neighbor_table[(size_t)b + (size_t)nb * k];
+            if (v < 0 || v > nb) return fail(c, LB_EINVAL, "level %d: neighbour table entry out of range", L.id);
+            int32_t id = v - 1;
+            if (id < 0 && L.id > 1) {
+                int64_t px = d->map_x[b] + k % 3 - 1, py = d->map_y[b] + (k / 3) % 3 - 1, pz = d->map_z[b] + k / 9 - 1;
+                if (px >= 1 && px <= dbx && py >= 1 && py <= dby && pz >= 1 && pz <= dbz) {
+                    auto it = ghost.find(key(px, py, pz));
+                    if (it == ghost.end()) {
+                        it = ghost.emplace(key(px, py, pz), (int32_t)gpos.size() / 3).first;
+                        gpos.push_back(px); gpos.push_back(py); gpos.push_back(pz);
+                    }
+                    id = nb + it->second;
+                }
+            }
+            nbr[(size_t)b * Q + k] = id;
+        }
+        if (nbr[(size_t)b * Q + 13] != b)
+            return fail(c, LB_EINVAL, "level %d: neighbour table centre entry of block %d is not itself", L.id, b + 1);
+        borg[(size_t)b * 3 + 0] = (d->map_x[b] - 1) * 8;
+        borg[(size_t)b * 3 + 1] = (d->map_y[b] - 1) * 8;
+        borg[(size_t)b * 3 + 2] = (d->map_z[b] - 1) * 8;
+    }
+    const int32_t ngh = (int32_t)(gpos.size() / 3);
+    for (int32_t g = 0; g < ngh; g++) {
+        const int64_t px = gpos[(size_t)g * 3], py = gpos[(size_t)g * 3 + 1], pz = gpos[(size_t)g * 3 + 2];
+        borg.push_back((int32_t)((px - 1) * 8));
+        borg.push_back((int32_t)((py - 1) * 8));
+        borg.push_back((int32_t)((pz - 1) * 8));
+        // ghost cell s serves direction k when s + c_k lies in a block of the level
+        for (int cl = 0; cl < NC; cl++) {
+            const int lx = cl & 7, ly = (cl >> 3) & 7, lz = cl >> 6;
+            uint32_t mask = 0;
+            for (int k = 0; k < Q; k++) {
+                const int nx = lx + k % 3 - 1, ny = ly + (k / 3) % 3 - 1, nz = lz + k / 9 - 1;
+                const int ox = nx < 0 ? -1 : (nx > 7 ? 1 : 0), oy = ny < 0 ? -1 : (ny > 7 ? 1 : 0),
+                          oz = nz < 0 ? -1 : (nz > 7 ? 1 : 0);
+                if (ox == 0 && oy == 0 && oz == 0) continue;
+                if (real.count(key(px + ox, py + oy, pz + oz))) mask |= 1u << k;
+            }
+            if (mask) {
+                gent.push_back((int32_t)(((nb + g) << 9) | cl));
+                gent.push_back((int32_t)mask);
+            }
+        }
+    }
+    L.ngh = ngh;
+    L.ngent = (int64_t)gent.size() / 2;
+    const size_t n4 = (size_t)nb * NC, n5 = ((size_t)nb + ngh) * NC * Q;
     for (int i = 0; i < 2; i++) {
         RC(dalloc(c, &L.f[i], n5));
         RC(dalloc(c, &L.rho[i], n4));
@@ -491,29 +598,31 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     RC(dalloc(c, &L.sponge, n4));
     if (d->wall_dist) RC(dalloc(c, &L.wall_dist, n4));
     RC(dalloc(c, &L.nbr, (size_t)nb * Q));
-    RC(dalloc(c, &L.borg, (size_t)nb * 3));
+    RC(dalloc(c, &L.borg, borg.size()));
+    RC(dalloc(c, &L.order, (size_t)nb));
+    RC(dalloc(c, &L.gent, gent.size()));
     const size_t ng = (size_t)d->ptr_dim_x * d->ptr_dim_y * d->ptr_dim_z;
     RC(dalloc(c, &L.grid, ng));
     L.pd[0] = d->ptr_dim_x;
     L.pd[1] = d->ptr_dim_y;
     L.pd[2] = d->ptr_dim_z;
+    if (!gent.empty()) RC(h2d(c, L.gent, gent.data(), gent.size()));
     if (nb > 0) {
-        std::vector<int32_t> nbr((size_t)nb * Q), borg((size_t)nb * 3);
-        for (int32_t b = 0; b < nb; b++) {
-            for (int k = 0; k < Q; k++) {
-                int v = d->neighbor_table[(size_t)b + (size_t)nb * k];
-                if (v < 0 || v > nb) return fail(c, LB_EINVAL, "level %d: neighbour table entry out of range", L.id);
-                nbr[(size_t)b * Q + k] = v - 1;
-            }
-            if (nbr[(size_t)b * Q + 13] != b)
-                return fail(c, LB_EINVAL, "level %d: neighbour table centre entry of block %d is not itself", L.id,
-                            b + 1);
-            borg[(size_t)b * 3 + 0] = (d->map_x[b] - 1) * 8;
-            borg[(size_t)b * 3 + 1] = (d->map_y[b] - 1) * 8;
-            borg[(size_t)b * 3 + 2] = (d->map_z[b] - 1) * 8;
-        }
         RC(h2d(c, L.nbr, nbr.data(), nbr.size()));
         RC(h2d(c, L.borg, borg.data(), borg.size()));
+        // processing order: blocks sorted by the Morton code of their coords
+        std::vector<std::pair<uint64_t, int32_t>> key((size_t)nb);
+        for (int32_t b = 0; b < nb; b++) {
+            uint64_t m = 0;
+            uint32_t v[3] = {(uint32_t)(d->map_x[b] - 1), (uint32_t)(d->map_y[b] - 1), (uint32_t)(d->map_z[b] - 1)};
+            for (int bit = 0; bit < 21; bit++)
+                for (int ax = 0; ax < 3; ax++) m |= (uint64_t)((v[ax] >> bit) & 1u) << (3 * bit + ax);
+            key[(size_t)b] = {m, b};
+        }
+        std::sort(key.begin(), key.end());
+        std::vector<int32_t> order((size_t)nb);
+        for (int32_t i = 0; i < nb; i++) order[(size_t)i] = key[(size_t)i].second;
+        RC(h2d(c, L.order, order.data(), order.size()));
         RC(h2d(c, L.obst, d->obstacle, n4));
         RC(h2d(c, L.sponge, d->sponge, n4));
         if (d->wall_dist) RC(h2d(c, L.wall_dist, d->wall_dist, n4));

@@ -31,6 +31,7 @@ struct CollideArgs {
     const float* wall_dist;  // may be null when wall_model == 0
     const int32_t* nbr;
     const int32_t* borg;
+    const int32_t* order;  // processing order (Morton-sorted block indices)
     int32_t nb;
     int32_t nxg, nyg, nzg;
     float tau;
@@ -64,6 +65,7 @@ struct ThermalArgs {
     const float* sponge;
     const int32_t* nbr;
     const int32_t* borg;
+    const int32_t* order;
     int32_t nb;
     int32_t nxg, nyg, nzg;
     float tau_g;
@@ -94,6 +96,28 @@ struct BouzidiLinks {
     const float* a;
     const float* b;
     float* tmp;
+};
+
+// Ghost blocks: in-domain positions next to a level's blocks that the level
+// does not hold.  Every link pulled from such a position takes the
+// coarse-to-fine interpolation, a function of the source cell and direction
+// only, so it is evaluated once per (ghost cell, direction) into the ghost
+// block's slots before the level's stream-collide.
+struct GhostArgs {
+    float* f;                  // populations being pulled (ghost blocks at index >= nb)
+    const int32_t* entries;    // [n][2]: (ghost block << 9 | cell, 27-bit direction mask)
+    int64_t n;
+    const int32_t* borg;       // block origins, ghost blocks included
+    const float* a_new;
+    const float* s_new;
+    const float* u_new;
+    const float* a_old;
+    const float* s_old;
+    const float* u_old;
+    const int32_t* pgrid;
+    int32_t pdx, pdy, pdz;
+    float tau_fine, tau_coarse, tw;
+    int32_t temporal, comp, thermal;
 };
 
 struct ForceArgs {
@@ -133,6 +157,7 @@ struct ForceArgs {
     hipError_t collide(const lbk::CollideArgs& a, hipStream_t s);                                        \
     hipError_t thermal(const lbk::ThermalArgs& a, hipStream_t s);                                        \
     hipError_t bouzidi(const lbk::BouzidiLinks& l, float* f, hipStream_t s);                             \
+    hipError_t ghost_fill(const lbk::GhostArgs& g, hipStream_t s);                                       \
     hipError_t init_level(float* f0, float* f1, float* rho0, float* rho1, float* v0, float* v1, int32_t nb, \
                           hipStream_t s);                                                                \
     hipError_t init_thermal(float* g0, float* g1, float* T0, float* T1, int32_t nb, float Tinit,         \

@@ -46,6 +46,14 @@ __host__ __device__ constexpr float WK(int k) {
 
 __device__ __forceinline__ size_t fidx(int b, int k, int c) { return ((size_t)b * Q + k) * NC + c; }
 
+// Global-address-space views of the argument pointers, so that loads and
+// stores compile to global_* (not flat_*) instructions.
+typedef __attribute__((address_space(1))) float GF;
+typedef __attribute__((address_space(1))) uint8_t GU8;
+__device__ __forceinline__ const GF* gp(const float* p) { return (const GF*)p; }
+__device__ __forceinline__ GF* gp(float* p) { return (GF*)p; }
+__device__ __forceinline__ const GU8* gp(const uint8_t* p) { return (const GU8*)p; }
+
 // Equilibria (src/collision/physics_utils.jl:19-82)
 __device__ __forceinline__ float feq2(float rho, float ux, float uy, float uz, float w, float cx, float cy, float cz) {
     float cu = cx * ux + cy * uy + cz * uz;
@@ -91,7 +99,7 @@ __device__ __forceinline__ float wk_rt(int k) {
 // interpolate_with_rescaling (physics_interpolation.jl:4-127) and its thermal
 // twin (:135-248).  Parent cell P sits at coordinate P; lower corner clamped
 // to >= 1 after the weights; missing corners take corner 000's tuple.
-__device__ __noinline__ float interp_rescale(const float* __restrict__ a_new, const float* __restrict__ s_new,
+__device__ __forceinline__ float interp_rescale(const float* __restrict__ a_new, const float* __restrict__ s_new,
                                              const float* __restrict__ u_new, const float* __restrict__ a_old,
                                              const float* __restrict__ s_old, const float* __restrict__ u_old,
                                              const int32_t* __restrict__ pgrid, int pdx, int pdy, int pdz,
@@ -172,7 +180,7 @@ __device__ __noinline__ float interp_rescale(const float* __restrict__ a_new, co
 }
 
 // Missing-neighbour cascade of stream_collide_cumulant! (kernel_cumulant.jl:117-159).
-__device__ __noinline__ float bc_value(const CollideArgs& a, int k, int b, int c, int sgx, int sgy, int sgz, int gy,
+__device__ __forceinline__ float bc_value(const CollideArgs& a, int k, int b, int c, int sgx, int sgy, int sgz, int gy,
                                        int gz) {
     float cx = (float)CX(k), cy = (float)CY(k), cz = (float)CZ(k), w = wk_rt(k);
     if (sgx < 1) {
@@ -187,29 +195,6 @@ __device__ __noinline__ float bc_value(const CollideArgs& a, int k, int b, int c
         return interp_rescale(a.pf_new, a.prho_new, a.pvel_new, a.pf_old, a.prho_old, a.pvel_old, a.pgrid, a.pdx,
                               a.pdy, a.pdz, sgx, sgy, sgz, k, a.tau, a.tau_parent, a.tw, a.temporal, a.comp, 0);
     return w;
-}
-
-// get_velocity_neighbor (physics_utils.jl:85-110): own-cell value when the
-// neighbour block is missing.
-__device__ __forceinline__ void vel_nb(const float* __restrict__ vin, const int* snbr, int b, int x, int y, int z,
-                                       int dx, int dy, int dz, float& u, float& v, float& w) {
-    int nx = x + dx, ny = y + dy, nz = z + dz;
-    int ox = nx < 0 ? -1 : (nx > 7 ? 1 : 0);
-    int oy = ny < 0 ? -1 : (ny > 7 ? 1 : 0);
-    int oz = nz < 0 ? -1 : (nz > 7 ? 1 : 0);
-    int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
-    int blk, c;
-    if (nbb >= 0) {
-        blk = nbb;
-        c = (nx & 7) + 8 * (ny & 7) + 64 * (nz & 7);
-    } else {
-        blk = b;
-        c = x + 8 * y + 64 * z;
-    }
-    size_t iv = (size_t)blk * 3 * NC + c;
-    u = vin[iv];
-    v = vin[iv + NC];
-    w = vin[iv + 2 * NC];
 }
 
 __device__ __forceinline__ void chim_fwd(float& fm, float& f0, float& fp, float u) {
@@ -232,13 +217,30 @@ __device__ __forceinline__ void chim_bwd(float& k0, float& k1, float& k2, float 
     k2 = fp;
 }
 
+// Work-group -> block schedule.  Work-groups are dealt round-robin to the 8
+// XCDs (each with its own L2); the swizzle gives every XCD one contiguous run
+// of the level's processing order (blocks sorted by Morton code, two
+// work-groups per block), so a block and most of its neighbours share an L2
+// and are processed close in time.  Returns false for padding work-groups.
+__device__ __forceinline__ bool schedule(const int32_t* __restrict__ order, int nb, int& b, int& half) {
+    const unsigned G = gridDim.x;            // multiple of 8
+    const unsigned w = blockIdx.x;
+    const unsigned logical = (w & 7u) * (G >> 3) + (w >> 3);
+    if (logical >= 2u * (unsigned)nb) return false;
+    b = order[logical >> 1];
+    half = (int)(logical & 1u);
+    return true;
+}
+
 // stream_collide_cumulant! (src/collision/kernel_cumulant.jl:29-627), with the
 // A/B buffers chosen by the caller and no f_post_collision copy (Bouzidi
 // reads f_out before it corrects it, see bouzidi_gather below).
 __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
     __shared__ int snbr[27];
-    const int b = blockIdx.x >> 1;
-    const int c = ((blockIdx.x & 1) << 8) | threadIdx.x;
+    __shared__ float sfix[27 * 256];
+    int b, half;
+    if (!schedule(a.order, a.nb, b, half)) return;
+    const int c = (half << 8) | threadIdx.x;
     if (threadIdx.x < 27) snbr[threadIdx.x] = a.nbr[(size_t)b * 27 + threadIdx.x];
     __syncthreads();
     const int x = c & 7, y = (c >> 3) & 7, z = c >> 6;
@@ -246,28 +248,71 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
     const int gy = a.borg[b * 3 + 1] + y + 1;
     const int gz = a.borg[b * 3 + 2] + z + 1;
     const size_t i4 = (size_t)b * NC + c;
-    const bool is_obs = a.obst[i4] != 0;
+    const bool is_obs = gp(a.obst)[i4] != 0;
 
-    // STEP 1: pull (kernel_cumulant.jl:86-168)
+    // STEP 1: pull (kernel_cumulant.jl:86-168).  All 27 loads are issued
+    // before any is used: a missing neighbour block only swaps in the cell's
+    // own slot as a harmless address, and the boundary cascade then replaces
+    // the value (miss mask), keeping the loads free of branches.
+    const GF* __restrict__ fin = gp(a.f_in);
     float A[27];
+    uint32_t miss = 0u;
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+        const int cx = CX(k), cy = CY(k), cz = CZ(k);
+        const int sx = x - cx, sy = y - cy, sz = z - cz;
+        const int ox = cx == 0 ? 0 : (sx < 0 ? -1 : (sx > 7 ? 1 : 0));
+        const int oy = cy == 0 ? 0 : (sy < 0 ? -1 : (sy > 7 ? 1 : 0));
+        const int oz = cz == 0 ? 0 : (sz < 0 ? -1 : (sz > 7 ? 1 : 0));
+        const int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+        const int blk = nbb >= 0 ? nbb : b;
+        const int cc = nbb >= 0 ? ((sx & 7) + 8 * (sy & 7) + 64 * (sz & 7)) : c;
+        A[k] = fin[fidx(blk, k, cc)];
+        miss |= (nbb < 0 ? 1u : 0u) << k;
+    }
+    // WALE stencil of the previous velocity (physics_utils.jl:112-123), loaded
+    // together with the populations.
+    float VN[6][3];
+    {
+        const GF* __restrict__ vin = gp(a.vel_in);
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            const int dx = n == 0 ? 1 : (n == 1 ? -1 : 0);
+            const int dy = n == 2 ? 1 : (n == 3 ? -1 : 0);
+            const int dz = n == 4 ? 1 : (n == 5 ? -1 : 0);
+            const int nx = x + dx, ny = y + dy, nz = z + dz;
+            const int ox = dx == 0 ? 0 : (nx < 0 ? -1 : (nx > 7 ? 1 : 0));
+            const int oy = dy == 0 ? 0 : (ny < 0 ? -1 : (ny > 7 ? 1 : 0));
+            const int oz = dz == 0 ? 0 : (nz < 0 ? -1 : (nz > 7 ? 1 : 0));
+            const int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+            const bool real = nbb >= 0 && nbb < a.nb;   // ghost blocks hold no velocity
+            const int blk = real ? nbb : b;
+            const int cc = real ? ((nx & 7) + 8 * (ny & 7) + 64 * (nz & 7)) : c;
+            const size_t iv = (size_t)blk * 3 * NC + cc;
+            VN[n][0] = vin[iv];
+            VN[n][1] = vin[iv + NC];
+            VN[n][2] = vin[iv + 2 * NC];
+        }
+    }
+    const float sp = gp(a.sponge)[i4];
+    const float wdist = a.wall_model == 1 ? gp(a.wall_dist)[i4] : 0.0f;
+    // Boundary cascade for sources in no block: one inlined copy of the
+    // cascade in a rolled loop over k, results passed through LDS (a
+    // register array cannot be indexed by a run-time k without spilling).
+    if (miss != 0u) {
+#pragma unroll 1
+        for (int k = 0; k < 27; k++)
+            if (miss & (1u << k))
+                sfix[k * 256 + threadIdx.x] = bc_value(a, k, b, c, gx - CX(k), gy - CY(k), gz - CZ(k), gy, gz);
+#pragma unroll
+        for (int k = 0; k < 27; k++)
+            if (miss & (1u << k)) A[k] = sfix[k * 256 + threadIdx.x];
+    }
     float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
 #pragma unroll
     for (int k = 0; k < 27; k++) {
-        constexpr int dummy = 0;
-        (void)dummy;
         const int cx = CX(k), cy = CY(k), cz = CZ(k);
-        int sx = x - cx, sy = y - cy, sz = z - cz;
-        int ox = cx == 0 ? 0 : (sx < 0 ? -1 : (sx > 7 ? 1 : 0));
-        int oy = cy == 0 ? 0 : (sy < 0 ? -1 : (sy > 7 ? 1 : 0));
-        int oz = cz == 0 ? 0 : (sz < 0 ? -1 : (sz > 7 ? 1 : 0));
-        int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
-        float val;
-        if (nbb >= 0) {
-            val = a.f_in[fidx(nbb, k, (sx & 7) + 8 * (sy & 7) + 64 * (sz & 7))];
-        } else {
-            val = bc_value(a, k, b, c, gx - cx, gy - cy, gz - cz, gy, gz);
-        }
-        A[k] = val;
+        const float val = A[k];
         rho += val;
         if (cx == 1) jx += val;
         if (cx == -1) jx += -val;
@@ -277,14 +322,16 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
         if (cz == -1) jz += -val;
     }
 
-    float* __restrict__ fo = a.f_out;
+    GF* __restrict__ fo = gp(a.f_out);
+    GF* __restrict__ vo = gp(a.vel_out);
+    GF* __restrict__ ro = gp(a.rho_out);
     if (is_obs) {
         // simple bounce-back of the streamed values (kernel_cumulant.jl:173-185)
         const size_t iv = (size_t)b * 3 * NC + c;
-        a.vel_out[iv] = 0.0f;
-        a.vel_out[iv + NC] = 0.0f;
-        a.vel_out[iv + 2 * NC] = 0.0f;
-        a.rho_out[i4] = 1.0f;
+        vo[iv] = 0.0f;
+        vo[iv + NC] = 0.0f;
+        vo[iv + 2 * NC] = 0.0f;
+        ro[i4] = 1.0f;
 #pragma unroll
         for (int k = 0; k < 27; k++) fo[fidx(b, k, c)] = A[OPP(k)];
         return;
@@ -294,7 +341,6 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
     rho = fmaxf(rho, 0.01f);
     const float inv_rho = 1.0f / rho;
     float ux = jx * inv_rho, uy = jy * inv_rho, uz = jz * inv_rho;
-    const float sp = a.sponge[i4];
     if (sp > 0.0f) {
         rho = rho * (1.0f - sp) + 1.0f * sp;
         ux = ux * (1.0f - sp) + a.u_inlet * sp;
@@ -311,7 +357,7 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
     }
     float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
     if (a.wall_model == 1) {
-        const float d = a.wall_dist[i4];
+        const float d = wdist;
         if (d > 0.0f && d < 10.0f) {
             float umag = sqrtf(ux * ux + uy * uy + uz * uz);
             float nu_visc = (a.tau - 0.5f) / 3.0f;
@@ -343,22 +389,18 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
     const float uze = uz + 0.5f * Fz * inv_rho;
     {
         const size_t iv = (size_t)b * 3 * NC + c;
-        a.vel_out[iv] = ux;
-        a.vel_out[iv + NC] = uy;
-        a.vel_out[iv + 2 * NC] = uz;
-        a.rho_out[i4] = rho;
+        vo[iv] = ux;
+        vo[iv + NC] = uy;
+        vo[iv + 2 * NC] = uz;
+        ro[i4] = rho;
     }
 
     // STEP 3: WALE on the previous velocity (kernel_cumulant.jl:264-310)
     float om1;
     {
-        float uE, vE, wE, uW, vW, wW, uN, vN, wN, uS, vS, wS, uT, vT, wT, uB, vB, wB;
-        vel_nb(a.vel_in, snbr, b, x, y, z, 1, 0, 0, uE, vE, wE);
-        vel_nb(a.vel_in, snbr, b, x, y, z, -1, 0, 0, uW, vW, wW);
-        vel_nb(a.vel_in, snbr, b, x, y, z, 0, 1, 0, uN, vN, wN);
-        vel_nb(a.vel_in, snbr, b, x, y, z, 0, -1, 0, uS, vS, wS);
-        vel_nb(a.vel_in, snbr, b, x, y, z, 0, 0, 1, uT, vT, wT);
-        vel_nb(a.vel_in, snbr, b, x, y, z, 0, 0, -1, uB, vB, wB);
+        const float uE = VN[0][0], vE = VN[0][1], wE = VN[0][2], uW = VN[1][0], vW = VN[1][1], wW = VN[1][2];
+        const float uN = VN[2][0], vN = VN[2][1], wN = VN[2][2], uS = VN[3][0], vS = VN[3][1], wS = VN[3][2];
+        const float uT = VN[4][0], vT = VN[4][1], wT = VN[4][2], uB = VN[5][0], vB = VN[5][1], wB = VN[5][2];
         float g11 = 0.5f * (uE - uW), g12 = 0.5f * (uN - uS), g13 = 0.5f * (uT - uB);
         float g21 = 0.5f * (vE - vW), g22 = 0.5f * (vN - vS), g23 = 0.5f * (vT - vB);
         float g31 = 0.5f * (wE - wW), g32 = 0.5f * (wN - wS), g33 = 0.5f * (wT - wB);
@@ -562,15 +604,17 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
     }
 }
 
+inline unsigned padded_groups(int nb) { return ((unsigned)nb * 2u + 7u) & ~7u; }
+
 hipError_t collide(const CollideArgs& a, hipStream_t s) {
     if (a.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(collide_kernel, dim3((unsigned)a.nb * 2), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(collide_kernel, dim3(padded_groups(a.nb)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // Thermal DDF (src/collision/kernel_thermal.jl:14-202)
-__device__ __noinline__ float thermal_bc(const ThermalArgs& a, int k, int b, int c, int sgx, int sgy, int sgz) {
+__device__ __forceinline__ float thermal_bc(const ThermalArgs& a, int k, int b, int c, int sgx, int sgy, int sgz) {
     float cx = (float)CX(k), cy = (float)CY(k), cz = (float)CZ(k), w = wk_rt(k);
     if (sgx < 1) return geq(a.T_inlet, a.u_inlet, 0.0f, 0.0f, w, cx, cy, cz);
     if (sgx > a.nxg) return geq(a.T_inlet, a.u_inlet, 0.0f, 0.0f, w, cx, cy, cz);
@@ -584,8 +628,10 @@ __device__ __noinline__ float thermal_bc(const ThermalArgs& a, int k, int b, int
 
 __global__ void __launch_bounds__(256) thermal_kernel(const ThermalArgs a) {
     __shared__ int snbr[27];
-    const int b = blockIdx.x >> 1;
-    const int c = ((blockIdx.x & 1) << 8) | threadIdx.x;
+    __shared__ float sfix[27 * 256];
+    int b, half;
+    if (!schedule(a.order, a.nb, b, half)) return;
+    const int c = (half << 8) | threadIdx.x;
     if (threadIdx.x < 27) snbr[threadIdx.x] = a.nbr[(size_t)b * 27 + threadIdx.x];
     __syncthreads();
     const int x = c & 7, y = (c >> 3) & 7, z = c >> 6;
@@ -594,24 +640,35 @@ __global__ void __launch_bounds__(256) thermal_kernel(const ThermalArgs a) {
     const int gz = a.borg[b * 3 + 2] + z + 1;
     const size_t i4 = (size_t)b * NC + c;
     const size_t iv = (size_t)b * 3 * NC + c;
-    const bool is_obs = a.obst[i4] != 0;
-    float ux = a.vel[iv], uy = a.vel[iv + NC], uz = a.vel[iv + 2 * NC];
+    const bool is_obs = gp(a.obst)[i4] != 0;
+    float ux = gp(a.vel)[iv], uy = gp(a.vel)[iv + NC], uz = gp(a.vel)[iv + 2 * NC];
+    const GF* __restrict__ gin = gp(a.g_in);
     float G[27];
-    float Tsum = 0.0f;
+    uint32_t miss = 0u;
 #pragma unroll
     for (int k = 0; k < 27; k++) {
         const int cx = CX(k), cy = CY(k), cz = CZ(k);
-        int sx = x - cx, sy = y - cy, sz = z - cz;
-        int ox = cx == 0 ? 0 : (sx < 0 ? -1 : (sx > 7 ? 1 : 0));
-        int oy = cy == 0 ? 0 : (sy < 0 ? -1 : (sy > 7 ? 1 : 0));
-        int oz = cz == 0 ? 0 : (sz < 0 ? -1 : (sz > 7 ? 1 : 0));
-        int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
-        float val;
-        if (nbb >= 0) val = a.g_in[fidx(nbb, k, (sx & 7) + 8 * (sy & 7) + 64 * (sz & 7))];
-        else val = thermal_bc(a, k, b, c, gx - cx, gy - cy, gz - cz);
-        Tsum += val;
-        G[k] = val;
+        const int sx = x - cx, sy = y - cy, sz = z - cz;
+        const int ox = cx == 0 ? 0 : (sx < 0 ? -1 : (sx > 7 ? 1 : 0));
+        const int oy = cy == 0 ? 0 : (sy < 0 ? -1 : (sy > 7 ? 1 : 0));
+        const int oz = cz == 0 ? 0 : (sz < 0 ? -1 : (sz > 7 ? 1 : 0));
+        const int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+        const int blk = nbb >= 0 ? nbb : b;
+        const int cc = nbb >= 0 ? ((sx & 7) + 8 * (sy & 7) + 64 * (sz & 7)) : c;
+        G[k] = gin[fidx(blk, k, cc)];
+        miss |= (nbb < 0 ? 1u : 0u) << k;
     }
+    if (miss != 0u) {
+#pragma unroll 1
+        for (int k = 0; k < 27; k++)
+            if (miss & (1u << k)) sfix[k * 256 + threadIdx.x] = thermal_bc(a, k, b, c, gx - CX(k), gy - CY(k), gz - CZ(k));
+#pragma unroll
+        for (int k = 0; k < 27; k++)
+            if (miss & (1u << k)) G[k] = sfix[k * 256 + threadIdx.x];
+    }
+    float Tsum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 27; k++) Tsum += G[k];
     if (is_obs) {
 #pragma unroll
         for (int k = 0; k < 27; k++) {
@@ -652,7 +709,132 @@ __global__ void __launch_bounds__(256) thermal_kernel(const ThermalArgs a) {
 
 hipError_t thermal(const ThermalArgs& a, hipStream_t s) {
     if (a.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(thermal_kernel, dim3((unsigned)a.nb * 2), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(thermal_kernel, dim3(padded_groups(a.nb)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Ghost-cell interpolation: interpolate_with_rescaling / its thermal twin
+// (physics_interpolation.jl:4-127, 135-248) for every direction a ghost cell
+// serves.  The corner geometry and the density/velocity part of the
+// trilinear blend depend on the cell only and are computed once; each
+// direction then blends its own population component.  Operation order per
+// value is that of interp_rescale above.
+__global__ void __launch_bounds__(256) ghost_kernel(const GhostArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int e0 = a.entries[2 * i];
+    const uint32_t mask = (uint32_t)a.entries[2 * i + 1];
+    const int gb = e0 >> 9, c = e0 & 511;
+    const int fgx = a.borg[gb * 3 + 0] + (c & 7) + 1;
+    const int fgy = a.borg[gb * 3 + 1] + ((c >> 3) & 7) + 1;
+    const int fgz = a.borg[gb * 3 + 2] + (c >> 6) + 1;
+    const float pxc = ((float)fgx - 0.5f) * 0.5f;
+    const float pyc = ((float)fgy - 0.5f) * 0.5f;
+    const float pzc = ((float)fgz - 0.5f) * 0.5f;
+    int px0 = (int)floorf(pxc), py0 = (int)floorf(pyc), pz0 = (int)floorf(pzc);
+    const int px1 = px0 + 1, py1 = py0 + 1, pz1 = pz0 + 1;
+    const float wx = pxc - (float)px0, wy = pyc - (float)py0, wz = pzc - (float)pz0;
+    px0 = max(1, px0);
+    py0 = max(1, py0);
+    pz0 = max(1, pz0);
+    const bool use_old = a.temporal == 1 && a.tw < 0.99f;
+    const float tw = a.tw;
+    int cb[8], cc[8];
+    bool ok[8];
+    float v[8][4];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int pgx = (j & 1) ? px1 : px0;
+        const int pgy = (j & 2) ? py1 : py0;
+        const int pgz = (j & 4) ? pz1 : pz0;
+        const int pbx = (pgx - 1) / BS + 1, pby = (pgy - 1) / BS + 1, pbz = (pgz - 1) / BS + 1;
+        ok[j] = false;
+        cb[j] = 0;
+        cc[j] = 0;
+        if (pbx >= 1 && pbx <= a.pdx && pby >= 1 && pby <= a.pdy && pbz >= 1 && pbz <= a.pdz) {
+            const int pb = a.pgrid[(pbx - 1) + a.pdx * ((pby - 1) + a.pdy * (pbz - 1))];
+            if (pb >= 0) {
+                ok[j] = true;
+                cb[j] = pb;
+                cc[j] = ((pgx - 1) % BS) + BS * ((pgy - 1) % BS) + BS * BS * ((pgz - 1) % BS);
+            }
+        }
+        if (ok[j]) {
+            const size_t i4 = (size_t)cb[j] * NC + cc[j];
+            const size_t iv = (size_t)cb[j] * 3 * NC + cc[j];
+            const float rn = a.s_new[i4], uxn = a.u_new[iv], uyn = a.u_new[iv + NC], uzn = a.u_new[iv + 2 * NC];
+            if (use_old) {
+                const float ro = a.s_old[i4], uxo = a.u_old[iv], uyo = a.u_old[iv + NC], uzo = a.u_old[iv + 2 * NC];
+                v[j][0] = ro * (1.0f - tw) + rn * tw;
+                v[j][1] = uxo * (1.0f - tw) + uxn * tw;
+                v[j][2] = uyo * (1.0f - tw) + uyn * tw;
+                v[j][3] = uzo * (1.0f - tw) + uzn * tw;
+            } else {
+                v[j][0] = rn; v[j][1] = uxn; v[j][2] = uyn; v[j][3] = uzn;
+            }
+        } else if (j == 0) {
+            v[0][0] = 1.0f; v[0][1] = 0.0f; v[0][2] = 0.0f; v[0][3] = 0.0f;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) v[j][q] = v[0][q];
+        }
+    }
+    float r[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float c00 = v[0][q] * (1.0f - wx) + v[1][q] * wx;
+        float c01 = v[4][q] * (1.0f - wx) + v[5][q] * wx;
+        float c10 = v[2][q] * (1.0f - wx) + v[3][q] * wx;
+        float c11 = v[6][q] * (1.0f - wx) + v[7][q] * wx;
+        float c0 = c00 * (1.0f - wy) + c10 * wy;
+        float c1 = c01 * (1.0f - wy) + c11 * wy;
+        r[q] = c0 * (1.0f - wz) + c1 * wz;
+    }
+    const float tc = a.tau_coarse - 0.5f;
+    const float tf = a.tau_fine - 0.5f;
+    float scale = 1.0f;
+    if (tc > 1.0e-6f) {
+        scale = tf / tc;
+        scale = scale > 100.0f ? 100.0f : (scale < 0.01f ? 0.01f : scale);
+    }
+    GF* __restrict__ fo = gp(a.f);
+#pragma unroll 1
+    for (int k = 0; k < 27; k++) {
+        if (!(mask & (1u << k))) continue;
+        const float w_k = wk_rt(k), cx = (float)CX(k), cy = (float)CY(k), cz = (float)CZ(k);
+        float fv[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (ok[j]) {
+                const float fn = a.a_new[fidx(cb[j], k, cc[j])];
+                if (use_old) {
+                    const float fo_ = a.a_old[fidx(cb[j], k, cc[j])];
+                    fv[j] = fo_ * (1.0f - tw) + fn * tw;
+                } else {
+                    fv[j] = fn;
+                }
+            } else {
+                fv[j] = j == 0 ? w_k : fv[0];
+            }
+        }
+        float c00 = fv[0] * (1.0f - wx) + fv[1] * wx;
+        float c01 = fv[4] * (1.0f - wx) + fv[5] * wx;
+        float c10 = fv[2] * (1.0f - wx) + fv[3] * wx;
+        float c11 = fv[6] * (1.0f - wx) + fv[7] * wx;
+        float c0 = c00 * (1.0f - wy) + c10 * wy;
+        float c1 = c01 * (1.0f - wy) + c11 * wy;
+        const float r0 = c0 * (1.0f - wz) + c1 * wz;
+        const float eq = a.thermal ? geq(r[0], r[1], r[2], r[3], w_k, cx, cy, cz)
+                                   : feq_auto(r[0], r[1], r[2], r[3], w_k, cx, cy, cz, a.comp);
+        const float neq = r0 - eq;
+        fo[fidx(gb, k, c)] = eq + neq * scale;
+    }
+}
+
+hipError_t ghost_fill(const GhostArgs& g, hipStream_t s) {
+    if (g.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ghost_kernel, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, s, g);
     return hipGetLastError();
 }
 
