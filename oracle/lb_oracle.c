@@ -934,10 +934,24 @@ static void timestep_level(orc_ctx* c, orc_level* L_, const orc_parent* pp, cons
     }
 }
 
+/* Optional trace of the recursion: (level 1-based, t_sub, tw) per substep. */
+static int* g_trace_lvl = NULL;
+static long long* g_trace_t = NULL;
+static double* g_trace_tw = NULL;
+static int g_trace_cap = 0, g_trace_n = 0;
+void orc_trace_set(int* lvl, long long* t, double* tw, int cap) {
+    g_trace_lvl = lvl; g_trace_t = t; g_trace_tw = tw; g_trace_cap = cap; g_trace_n = 0;
+}
+int orc_trace_count(void) { return g_trace_n; }
+
 /* recursive_step! / recursive_step_temporal!: src/solver/solver_control.jl:6-197 */
 static void recursive_step(orc_ctx* c, int lvl, int t_sub, const orc_parent* pp, const orc_params* prm, real u,
                            real tw) {
     if (lvl >= c->n_levels) return;
+    if (g_trace_cap > 0 && g_trace_n < g_trace_cap) {
+        g_trace_lvl[g_trace_n] = lvl + 1; g_trace_t[g_trace_n] = t_sub; g_trace_tw[g_trace_n] = (double)tw;
+        g_trace_n++;
+    }
     orc_level* L_ = &c->lv[lvl];
     size_t n4 = (size_t)512 * L_->nb, n5 = n4 * 27;
     real *f_in, *f_out, *vin, *vout, *gin = NULL, *gout = NULL;
@@ -1055,8 +1069,8 @@ int orc_put(orc_ctx* c, int lvl, int which, int t_last, const double* in) {
 /* Collide one isolated cell whose 27 pulled populations are `fin`, with
  * uniform neighbour velocity `u_nb` (so the WALE gradient is zero).  Returns
  * post-collision populations in fout, rho/u in macro[4].                   */
-int orc_collide_single(const double* fin, const double* u_nb, float tau, const orc_params* prm, double sponge,
-                       double wall_dist, float u_inlet, double* fout, double* macro) {
+int orc_collide_single(const double* fin, const double* u_nb, const double* grad, float tau, const orc_params* prm,
+                       double sponge, double wall_dist, float u_inlet, double* fout, double* macro) {
     build_lattice();
     orc_level Lv;
     memset(&Lv, 0, sizeof(Lv));
@@ -1075,7 +1089,13 @@ int orc_collide_single(const double* fin, const double* u_nb, float tau, const o
     memset(obst, 0, sizeof(obst));
     for (int i = 0; i < 512; i++) {
         spg[i] = (real)sponge; wd[i] = (real)wall_dist; rho[i] = 1;
-        vin[i] = (real)u_nb[0]; vin[i + 512] = (real)u_nb[1]; vin[i + 1024] = (real)u_nb[2];
+        /* previous velocity u_nb + G (x - x0), x0 = the probed cell (4,4,4) */
+        double px = (double)(i % 8) - 3.0, py = (double)((i / 8) % 8) - 3.0, pz = (double)(i / 64) - 3.0;
+        for (int d = 0; d < 3; d++) {
+            double v = u_nb[d];
+            if (grad) v += grad[3 * d + 0] * px + grad[3 * d + 1] * py + grad[3 * d + 2] * pz;
+            vin[i + 512 * d] = (real)v;
+        }
         for (int k = 0; k < 27; k++) fi[i + 512 * k] = (real)fin[k];
     }
     Lv.obst = obst; Lv.sponge = spg; Lv.wall_dist = wd; Lv.rho = rho; Lv.tau = (real)tau;
@@ -1150,4 +1170,149 @@ double orc_interp_probe(orc_ctx* c, int plvl, int t_last, int fgx, int fgy, int 
     pp.vel_old = P->vel_old ? P->vel_old : pp.vel_new;
     pp.tau_parent = P->tau;
     return (double)interp_rescale(&pp, 0, fgx, fgy, fgz, k, (real)tau_fine, (real)tw, temporal, comp);
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* Surface forces: map_stresses_kernel! + integrate_forces_kernel!
+ * (src/forces/surface.jl:143-507, 702-762) on the finest level, per-triangle
+ * terms in FP32 as the reference computes them, summed in FP64 in triangle
+ * order (the reference's FP32 atomics have no defined order).
+ * Reads level.rho and level.vel -- the vel buffer of even-numbered storage
+ * (blocks.jl: `vel`), exactly the arrays surface.jl:784 passes.            */
+static int f_cell_at(const orc_level* P, int gx, int gy, int gz, int* b, size_t* i4) {
+    if (gx < 1 || gy < 1 || gz < 1) return 0;
+    int bx = (gx - 1) / BS + 1, by = (gy - 1) / BS + 1, bz = (gz - 1) / BS + 1;
+    if (bx < 1 || bx > P->pdx || by < 1 || by > P->pdy || bz < 1 || bz > P->pdz) return 0;
+    int id = P->bptr[(size_t)(bx - 1) + (size_t)P->pdx * ((by - 1) + (size_t)P->pdy * (bz - 1))];
+    if (id <= 0) return 0;
+    *b = id;
+    *i4 = I4(P, (gx - 1) % BS + 1, (gy - 1) % BS + 1, (gz - 1) % BS + 1, id);
+    return 1;
+}
+
+int orc_forces(const orc_ctx* c, int t_last, const float* cxt, const float* cyt, const float* czt, const float* nxt,
+               const float* nyt, const float* nzt, const float* area, long long n_tri, const float* off,
+               const float* ref, float dx, float pressure_scale, float stress_scale, int search_radius,
+               float c_wale, float nu_bg, double* out9, float* maps) {
+    (void)t_last;
+    const orc_level* P = &c->lv[c->n_levels - 1];
+    size_t n4 = (size_t)512 * P->nb;
+    const real* rho_a = P->rho;
+    const real* vel_a = P->vel;
+    for (int j = 0; j < 9; j++) out9[j] = 0.0;
+    float tau_mol = (float)P->tau;
+    for (long long i = 0; i < n_tri; i++) {
+        float tx = cxt[i] + off[0], ty = cyt[i] + off[1], tz = czt[i] + off[2];
+        float n_x = nxt[i], n_y = nyt[i], n_z = nzt[i];
+        float brho = 1.0f, bux = 0.0f, buy = 0.0f, buz = 0.0f, bwd = 1.0f, bq = 0.0f;
+        int found = 0;
+        for (int pi = 1; pi <= search_radius; pi++) {
+            float dc = (float)pi + 0.5f;
+            float px = tx + n_x * dc * dx, py = ty + n_y * dc * dx, pz = tz + n_z * dc * dx;
+            /* trilinear_sample_flow (surface.jl:224-346) */
+            float inv_dx = 1.0f / dx;
+            float fx = px * inv_dx - 0.5f, fy = py * inv_dx - 0.5f, fz = pz * inv_dx - 0.5f;
+            int ix0 = (int)floorf(fx) + 1, iy0 = (int)floorf(fy) + 1, iz0 = (int)floorf(fz) + 1;
+            float wx = fx - (float)(ix0 - 1), wy = fy - (float)(iy0 - 1), wz = fz - (float)(iz0 - 1);
+            wx = wx < 0.0f ? 0.0f : (wx > 1.0f ? 1.0f : wx);
+            wy = wy < 0.0f ? 0.0f : (wy > 1.0f ? 1.0f : wy);
+            wz = wz < 0.0f ? 0.0f : (wz > 1.0f ? 1.0f : wz);
+            float w0x = 1.0f - wx, w1x = wx, w0y = 1.0f - wy, w1y = wy, w0z = 1.0f - wz, w1z = wz;
+            float racc = 0, uacc = 0, vacc = 0, zacc = 0, wacc = 0;
+            for (int q = 0; q < 8; q++) {
+                int dxi = q & 1, dyi = (q >> 1) & 1, dzi = (q >> 2) & 1;
+                float wk = (dxi ? w1x : w0x) * (dyi ? w1y : w0y) * (dzi ? w1z : w0z);
+                int b;
+                size_t j4;
+                if (f_cell_at(P, ix0 + dxi, iy0 + dyi, iz0 + dzi, &b, &j4) && !P->obst[j4]) {
+                    racc += wk * (float)rho_a[j4];
+                    uacc += wk * (float)vel_a[j4];
+                    vacc += wk * (float)vel_a[j4 + n4];
+                    zacc += wk * (float)vel_a[j4 + 2 * n4];
+                    wacc += wk;
+                }
+            }
+            int ok = wacc > 0.01f;
+            float r = 1.0f, u = 0.0f, v = 0.0f, w = 0.0f, wq = 0.0f;
+            if (ok) {
+                float iw = 1.0f / wacc;
+                r = racc * iw; u = uacc * iw; v = vacc * iw; w = zacc * iw; wq = wacc;
+            }
+            if (ok && wq >= 0.5f) {
+                brho = r; bux = u; buy = v; buz = w; bwd = dc; bq = wq; found = 1;
+                break;
+            } else if (ok && wq > bq) {
+                brho = r; bux = u; buy = v; buz = w; bwd = dc; bq = wq; found = 1;
+            }
+        }
+        if (!found) {
+            float gxf = tx / dx, gyf = ty / dx, gzf = tz / dx;
+            int g_x = (int)floorf(gxf) + 1, g_y = (int)floorf(gyf) + 1, g_z = (int)floorf(gzf) + 1;
+            float best = 1e10f;
+            for (int rad = 1; rad <= search_radius; rad++) {
+                if (found) break;
+                for (int dz = -rad; dz <= rad; dz++)
+                    for (int dy = -rad; dy <= rad; dy++)
+                        for (int ddx = -rad; ddx <= rad; ddx++) {
+                            int shell = abs(ddx) == rad || abs(dy) == rad || abs(dz) == rad;
+                            if (!shell) continue;
+                            int cgx = g_x + ddx, cgy = g_y + dy, cgz = g_z + dz, b;
+                            size_t j4;
+                            if (!f_cell_at(P, cgx, cgy, cgz, &b, &j4) || P->obst[j4]) continue;
+                            float ccx = ((float)cgx - 0.5f) * dx, ccy = ((float)cgy - 0.5f) * dx,
+                                  ccz = ((float)cgz - 0.5f) * dx;
+                            float dxp = ccx - tx, dyp = ccy - ty, dzp = ccz - tz;
+                            float dn = dxp * n_x + dyp * n_y + dzp * n_z;
+                            if (dn <= 0.0f) continue;
+                            float dsq = dxp * dxp + dyp * dyp + dzp * dzp;
+                            if (dsq < best) {
+                                best = dsq;
+                                brho = (float)rho_a[j4];
+                                bux = (float)vel_a[j4]; buy = (float)vel_a[j4 + n4]; buz = (float)vel_a[j4 + 2 * n4];
+                                bwd = dn / dx;
+                                found = 1;
+                            }
+                        }
+            }
+        }
+        float p = 0.0f, tx_ = 0.0f, ty_ = 0.0f, tz_ = 0.0f;
+        if (found) {
+            float wd = bwd > 0.5f ? bwd : 0.5f;
+            float pg = (brho - 1.0f) / 3.0f;
+            p = pg * pressure_scale;
+            float und = bux * n_x + buy * n_y + buz * n_z;
+            float utx = bux - und * n_x, uty = buy - und * n_y, utz = buz - und * n_z;
+            float utm = sqrtf(utx * utx + uty * uty + utz * utz);
+            float nu_mol = (tau_mol - 0.5f) / 3.0f;
+            float nu_sgs = 0.0f;
+            if (c_wale > 0.0f && wd > 0.01f && utm > 1.0e-10f) {
+                float S = utm / wd;
+                nu_sgs = (c_wale * c_wale) * S;
+            }
+            nu_sgs = nu_sgs > nu_bg ? nu_sgs : nu_bg;
+            float nu_eff = nu_mol + nu_sgs;
+            if (utm > 1.0e-10f && wd > 0.01f) {
+                float tl = brho * nu_eff * utm / wd;
+                float tp = tl * stress_scale;
+                tx_ = (utx / utm) * tp;
+                ty_ = (uty / utm) * tp;
+                tz_ = (utz / utm) * tp;
+            }
+        }
+        if (maps) {
+            maps[i] = p; maps[i + n_tri] = tx_; maps[i + 2 * n_tri] = ty_; maps[i + 3 * n_tri] = tz_;
+        }
+        float A = area[i];
+        float dFpx = -p * n_x * A, dFpy = -p * n_y * A, dFpz = -p * n_z * A;
+        float dFvx = tx_ * A, dFvy = ty_ * A, dFvz = tz_ * A;
+        float dFx = dFpx + dFvx, dFy = dFpy + dFvy, dFz = dFpz + dFvz;
+        float rx = (cxt[i] + off[0]) - ref[0], ry = (cyt[i] + off[1]) - ref[1], rz = (czt[i] + off[2]) - ref[2];
+        out9[0] += (double)dFpx; out9[1] += (double)dFpy; out9[2] += (double)dFpz;
+        out9[3] += (double)dFvx; out9[4] += (double)dFvy; out9[5] += (double)dFvz;
+        out9[6] += (double)(ry * dFz - rz * dFy);
+        out9[7] += (double)(rz * dFx - rx * dFz);
+        out9[8] += (double)(rx * dFy - ry * dFx);
+    }
+    return 0;
 }

@@ -53,8 +53,12 @@ def lib(precision: str = "f32") -> C.CDLL:
         L.orc_put.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
         L.orc_ramp_velocity.restype = C.c_float
         L.orc_ramp_velocity.argtypes = [C.c_float, C.c_int, C.c_int]
-        L.orc_collide_single.argtypes = [dp, dp, C.c_float, C.POINTER(OrcParams), C.c_double, C.c_double,
+        L.orc_collide_single.argtypes = [dp, dp, dp, C.c_float, C.POINTER(OrcParams), C.c_double, C.c_double,
                                          C.c_float, dp, dp]
+        L.orc_trace_set.argtypes = [ip, C.POINTER(C.c_longlong), dp, C.c_int]
+        L.orc_trace_count.restype = C.c_int
+        L.orc_forces.argtypes = [vp, C.c_int, fp_, fp_, fp_, fp_, fp_, fp_, fp_, C.c_longlong, fp_, fp_, C.c_float,
+                                 C.c_float, C.c_float, C.c_int, C.c_float, C.c_float, dp, fp_]
         L.orc_chimera_roundtrip.argtypes = [dp, C.c_double, C.c_double, C.c_double, dp]
         L.orc_chimera_forward.argtypes = [dp, C.c_double, C.c_double, C.c_double, dp]
         L.orc_feq.restype = C.c_double
@@ -171,3 +175,34 @@ class Oracle:
             rc = self.L.orc_put(self.ctx, level_id - 1, which, t, _p(a, C.c_double))
             if rc != 0:
                 raise RuntimeError("orc_put failed")
+
+
+def oracle_forces(orc: "Oracle", cfg, params, mesh, level_dx: float):
+    """compute_aerodynamics! restated (FP64 ordered sum): dict like Solver.compute_aerodynamics."""
+    f32 = np.float32
+    arr = [np.ascontiguousarray(a, dtype=np.float32) for a in
+           (mesh.centers[:, 0], mesh.centers[:, 1], mesh.centers[:, 2], mesh.normals[:, 0], mesh.normals[:, 1],
+            mesh.normals[:, 2], mesh.areas)]
+    off = np.array(params.mesh_offset, dtype=np.float32)
+    ref = np.array(params.moment_center, dtype=np.float32)
+    ps = float(f32(params.rho_physical * params.velocity_scale * params.velocity_scale))
+    out = np.zeros(9)
+    maps = np.zeros(4 * mesh.n, dtype=np.float32)
+    orc.L.orc_forces(orc.ctx, 0, *[_p(a, C.c_float) for a in arr], mesh.n, _p(off, C.c_float), _p(ref, C.c_float),
+                     float(f32(level_dx)), ps, ps, 5, cfg.c_wale, cfg.nu_sgs_background, _p(out, C.c_double),
+                     _p(maps, C.c_float))
+    Fxp, Fyp, Fzp, Fxv, Fyv, Fzv, Mx, My, Mz = out
+    if cfg.symmetric_analysis:
+        Fxp *= 2.0; Fzp *= 2.0; Fxv *= 2.0; Fzv *= 2.0; My *= 2.0
+        Fyp = Fyv = Mx = Mz = 0.0
+    q_inf = 0.5 * params.rho_physical * params.u_physical ** 2
+    F_ref = q_inf * params.reference_area
+    M_ref = F_ref * params.reference_chord
+    res = {"Fx_pressure": Fxp, "Fy_pressure": Fyp, "Fz_pressure": Fzp, "Fx_viscous": Fxv, "Fy_viscous": Fyv,
+           "Fz_viscous": Fzv, "Mx": Mx, "My": My, "Mz": Mz, "Fx": Fxp + Fxv, "Fy": Fyp + Fyv, "Fz": Fzp + Fzv}
+    res["Cd"] = res["Fx"] / F_ref if F_ref > 1e-10 else 0.0
+    res["Cl"] = res["Fz"] / F_ref if F_ref > 1e-10 else 0.0
+    res["Cs"] = res["Fy"] / F_ref if F_ref > 1e-10 else 0.0
+    res["Cmx"], res["Cmy"], res["Cmz"] = ((Mx / M_ref, My / M_ref, Mz / M_ref) if M_ref > 1e-10 else (0.0, 0.0, 0.0))
+    res["maps"] = maps.reshape(4, mesh.n)
+    return res
