@@ -128,3 +128,107 @@ def test_fast_one_step_and_n_steps(hip_device):
         assert mx <= 5e-5, f"40 steps {k}: max |diff| {mx}"
         if k[1] != "f":
             assert rel <= 1e-5, f"40 steps {k}: L2-rel {rel}"
+
+
+@pytest.mark.gpu
+def test_strict_thermal_ddf_bitexact(hip_device):
+    """Thermal DDF (kernel_thermal.jl) + O(u^4) equilibrium, 2 levels, Bouzidi
+    on g, strict build: g, T and the flow fields bit-identical to the oracle."""
+    cfg, mesh = cases.tiny_sphere(8, 2, ddf=True)
+    cfg = cfg.replace(t_inlet=1.05, t_initial=1.0)
+    grids, params, mesh = build_case(cfg, mesh)
+    assert cfg.ddf_enabled and grids[-1].tau_g > 0.5
+    s, o = run_pair(cfg, grids, params, mesh, steps=8, strict=True)
+    res = compare(s, o, grids, fields=("f", "rho", "vel", "g", "T"))
+    print(res)
+    for k, (mx, nz, rel) in res.items():
+        assert mx == 0.0, f"{k}: max |diff| {mx} over {nz} entries"
+
+
+@pytest.mark.gpu
+def test_forces_match_oracle(hip_device):
+    """compute_aerodynamics! on the finest level: per-triangle stresses equal
+    the oracle's (same FP32 arithmetic); the nine sums are FP64 in a fixed
+    order on both sides, so Cd/Cl agree to summation-order rounding."""
+    from oracle import oracle_forces
+    cfg, mesh = cases.tiny_sphere(8, 2)
+    grids, params, mesh = build_case(cfg, mesh)
+    s, o = run_pair(cfg, grids, params, mesh, steps=16, strict=True)
+    fg = s.compute_aerodynamics(maps=True)
+    fo = oracle_forces(o, cfg, params, mesh, grids[-1].dx)
+    for i in range(4):
+        assert np.array_equal(fg["maps"][i], fo["maps"][i]), f"map {i} differs"
+    for key in ("Fx", "Fy", "Fz", "Mx", "My", "Mz", "Cd", "Cl", "Cs"):
+        a, b = fg[key], fo[key]
+        assert abs(a - b) <= 1e-9 * max(1.0, abs(b)), f"{key}: {a} vs {b}"
+    assert abs(fg["Cd"]) > 1e-6
+
+
+@pytest.mark.gpu
+def test_uniform_pressure_closed_mesh_zero_force(hip_device):
+    """Sum n_i A_i = 0 on a closed mesh: a uniform density (any value) and zero
+    velocity give zero pressure force to rounding."""
+    cfg, mesh = cases.tiny_sphere(8, 2)
+    grids, params, mesh = build_case(cfg, mesh)
+    s = Solver(cfg, grids, params, mesh, strict_fp=True)
+    s.init_equilibrium()
+    g = grids[-1]
+    s.upload(g.level_id, "rho", np.full(g.n_cells, 1.01, dtype=np.float32))
+    f = s.compute_aerodynamics()
+    p = (1.01 - 1.0) / 3.0 * params.rho_physical * params.velocity_scale ** 2
+    scale = p * mesh.areas.sum()
+    assert abs(f["Fx_pressure"]) < 1e-5 * scale and abs(f["Fz_pressure"]) < 1e-5 * scale
+
+
+@pytest.mark.gpu
+def test_sphere_three_level_fast_vs_fp64_oracle(hip_device):
+    """The 3-level sphere workload (64x32x32 coarse, 1.34 M cells), fast build,
+    16 coarse steps from rest: populations within 5e-5 of the FP32 mirror, and
+    the error report against the FP64 oracle (fields and Cd)."""
+    from oracle import oracle_forces
+    cfg, mesh = cases.sphere_re1e3()
+    cfg = cfg.replace(ramp_steps=8)            # reach u_lat = 0.05 within the run
+    grids, params, mesh = build_case(cfg, mesh)
+    s, o = run_pair(cfg, grids, params, mesh, steps=16, strict=False, perturb=False, batch=8)
+    res = compare(s, o, grids)
+    print(res)
+    for k, (mx, nz, rel) in res.items():
+        assert mx <= 5e-5, f"{k}: max |diff| {mx}"
+    o64 = Oracle(cfg, grids, params, "f64")
+    o64.init_equilibrium()
+    for t in (1, 9):
+        o64.step_batch(t, 8, ramp_velocity(cfg.u_lattice, t + 7, cfg.ramp_steps))
+    u_in = cfg.u_lattice
+    for g in grids:
+        a = s.download(g.level_id, "vel").astype(np.float64)
+        b = o64.get(g.level_id, "vel")
+        rms = np.sqrt(np.mean((a - b) ** 2)) / u_in
+        print(f"level {g.level_id}: u-field RMS error vs FP64 oracle {rms:.3e} u_in, "
+              f"max {np.abs(a - b).max() / u_in:.3e} u_in")
+        assert rms < 1e-4
+
+
+@pytest.mark.gpu
+def test_cd_developed_flow_vs_fp64_oracle(hip_device):
+    """Cd and Cl of a developed flow (2-level sphere, inlet ramp of 8 steps,
+    240 coarse steps so the flow has passed the body several times): fast
+    GPU build vs the FP64 oracle within 0.5 % of Cd (the stated tolerance)."""
+    from oracle import oracle_forces
+    cfg, mesh = cases.tiny_sphere(8, 2)
+    cfg = cfg.replace(ramp_steps=8)
+    grids, params, mesh = build_case(cfg, mesh)
+    s = Solver(cfg, grids, params, mesh, strict_fp=False)
+    o64 = Oracle(cfg, grids, params, "f64")
+    s.init_equilibrium()
+    o64.init_equilibrium()
+    for t in range(1, 241, 8):
+        u = ramp_velocity(cfg.u_lattice, t + 7, cfg.ramp_steps)
+        s.execute_timestep_batch(t, 8, u)
+        o64.step_batch(t, 8, u)
+    s.synchronize()
+    fg = s.compute_aerodynamics()
+    f64 = oracle_forces(o64, cfg, params, mesh, grids[-1].dx)
+    print("Cd gpu", fg["Cd"], "fp64", f64["Cd"], "Cl gpu", fg["Cl"], "fp64", f64["Cl"])
+    assert abs(f64["Cd"]) > 0.05
+    assert abs(fg["Cd"] - f64["Cd"]) <= 5e-3 * abs(f64["Cd"])
+    assert abs(fg["Cl"] - f64["Cl"]) <= 5e-3 * abs(f64["Cd"])
