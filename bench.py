@@ -13,8 +13,8 @@ value = MLUPS_true (every substep's cell updates counted) over the timed
 steps, with the populations resident in HBM.  roofline = the finest level's
 stream-collide kernel (the dominant one): algorithmic bytes per launch
 (253 B per cell update: 2x27x4 B populations + 33 B macroscopic/flags + 4 B
-wall distance) / its average launch time from HIP events on the library's
-stream.  cpu_baseline = the C oracle (the restated reference algorithm) on
+wall distance; 145 B with --storage fp16/bf16, populations as 16-bit offsets
+f - w) / its average launch time from HIP events on the library's stream.  cpu_baseline = the C oracle (the restated reference algorithm) on
 the host cores over one coarse step of the same workload.
 """
 from __future__ import annotations
@@ -29,7 +29,10 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-B_ALG_FP32_WMLES = 253      # bytes per cell update, FP32 populations + wall model
+# bytes per cell update with the wall model: 2 x 27 populations + 33 B of
+# macroscopic fields/flags + 4 B wall distance
+B_ALG_WMLES = {"fp32": 2 * 27 * 4 + 37, "fp16": 2 * 27 * 2 + 37, "bf16": 2 * 27 * 2 + 37}
+STORAGE_CODE = {"fp32": 0, "fp16": 1, "bf16": 2}
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 
 
@@ -44,6 +47,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--resolution", type=int, default=160)
     ap.add_argument("--strict", action="store_true", help="use the no-FMA build")
+    ap.add_argument("--storage", default="fp32", choices=sorted(B_ALG_WMLES),
+                    help="population storage (fp16/bf16: offsets f - w, FP32 arithmetic)")
     ap.add_argument("--cpu-steps", type=int, default=1, help="coarse steps timed for the CPU baseline (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -62,7 +67,8 @@ def main() -> int:
     log(f"setup {time.time() - t0:.1f}s: levels {len(grids)}, cells {[f'{c / 1e6:.2f}M' for c in n_cells]}, "
         f"bouzidi cells {grids[-1].n_boundary_cells}, tau {params.tau_levels}")
 
-    solver = Solver(cfg, grids, params, mesh, strict_fp=args.strict)
+    solver = Solver(cfg, grids, params, mesh, strict_fp=args.strict, storage=STORAGE_CODE[args.storage])
+    b_alg = B_ALG_WMLES[args.storage]
     solver.init_equilibrium()
     batch = cfg.gpu_async_depth
 
@@ -107,7 +113,8 @@ def main() -> int:
     # HBM bytes per launch of the same kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes (scaled by a copy of known size)
     traffic = None
-    prof = ROOT / "profiles" / "collide_cube_re1e5_52M.json"
+    prof = ROOT / "profiles" / ("collide_cube_re1e5_52M.json" if args.storage == "fp32"
+                                else f"collide_cube_re1e5_52M_{args.storage}.json")
     if prof.exists() and not args.strict and args.resolution == 160:
         pj = json.loads(prof.read_text())
         if pj.get("cells") == grids[-1].n_cells:
@@ -116,10 +123,10 @@ def main() -> int:
     m_true, m_ref = mlups(grids, args.steps, elapsed)
     ms_k, n_k = kern[finest]
     avg_ms = ms_k / max(n_k, 1)
-    bytes_launch = grids[-1].n_cells * B_ALG_FP32_WMLES
+    bytes_launch = grids[-1].n_cells * b_alg
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     per_level = {str(lid): {"launches": n, "avg_ms": round(ms / max(n, 1), 4),
-                            "GBps": round(grids[lid - 1].n_cells * B_ALG_FP32_WMLES / (ms / max(n, 1) * 1e-3) / 1e9, 1)
+                            "GBps": round(grids[lid - 1].n_cells * b_alg / (ms / max(n, 1) * 1e-3) / 1e9, 1)
                             if n else None} for lid, (ms, n) in kern.items()}
     log(f"timed {args.steps} coarse steps in {elapsed:.3f}s: MLUPS_true {m_true:.0f}, MLUPS_ref {m_ref:.0f}; "
         f"finest collide {avg_ms:.3f} ms/launch = {achieved:.0f} GB/s; Cd {forces['Cd']:.4f}")
@@ -161,7 +168,8 @@ def main() -> int:
                                f"resolution {args.resolution}",
                    "cells_per_level": n_cells, "lu_per_step": sum(c * 2 ** i for i, c in enumerate(n_cells)),
                    "mlups_ref": round(m_ref, 1), "build": "strict" if args.strict else "fast",
-                   "hbm_roofline_pct_step": round(m_true * 1e6 * B_ALG_FP32_WMLES / (HBM_PEAK_GBS * 1e9) * 100, 2),
+                   "population_storage": args.storage, "bytes_per_lu": b_alg,
+                   "hbm_roofline_pct_step": round(m_true * 1e6 * b_alg / (HBM_PEAK_GBS * 1e9) * 100, 2),
                    "per_level_collide": per_level, "Cd": round(forces["Cd"], 5)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -200,6 +200,12 @@ int64_t lb_device_bytes(const lb_ctx* ctx);
 int lb_profile_enable(lb_ctx* ctx, int enable);
 int lb_profile_read(lb_ctx* ctx, int level_id, double* ms, int64_t* launches);
 
+/* HBM bandwidth probe (no counterpart in the reference): streams `bytes`
+ * with 16-byte accesses; gbps_out[0] copy (read + write counted),
+ * [1] read-only, [2] write-only, each averaged over `reps` launches.
+ * The practical ceiling reported beside the nominal 8 TB/s.               */
+int lb_bandwidth_probe(int device, int64_t bytes, int reps, double* gbps_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,7 +59,8 @@ def _newer(target: Path, deps: list[Path]) -> bool:
 def build_hip(force: bool = True) -> Path:
     OUT.mkdir(exist_ok=True)
     lib = OUT / "libludwig_hip.so"
-    deps = [CSRC / "lb_kernels.hip", CSRC / "lb_api.hip", CSRC / "lb_kernels.h", INC / "ludwig_hip.h"]
+    deps = [CSRC / "lb_kernels.hip", CSRC / "lb_api.hip", CSRC / "lb_probe.hip", CSRC / "lb_kernels.h",
+            INC / "ludwig_hip.h"]
     if not force and _newer(lib, deps):
         return lib
     objs = []
@@ -68,9 +69,10 @@ def build_hip(force: bool = True) -> Path:
         _run([HIPCC, *DEVICE_FLAGS, f"-ffp-contract={contract}", f"-DLBK_NS={ns}", "-c",
               str(CSRC / "lb_kernels.hip"), "-o", str(obj)])
         objs.append(obj)
-    api = OUT / "lb_api.o"
-    _run([HIPCC, *DEVICE_FLAGS, "-ffp-contract=off", "-c", str(CSRC / "lb_api.hip"), "-o", str(api)])
-    objs.append(api)
+    for src in ("lb_api", "lb_probe"):
+        obj = OUT / f"{src}.o"
+        _run([HIPCC, *DEVICE_FLAGS, "-ffp-contract=off", "-c", str(CSRC / f"{src}.hip"), "-o", str(obj)])
+        objs.append(obj)
     tmp = OUT / "libludwig_hip.so.tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     shutil.move(str(tmp), str(lib))

@@ -39,12 +39,12 @@ constexpr int Q = 27;
 struct Kern {
     hipError_t (*collide)(const lbk::CollideArgs&, hipStream_t);
     hipError_t (*thermal)(const lbk::ThermalArgs&, hipStream_t);
-    hipError_t (*bouzidi)(const lbk::BouzidiLinks&, float*, hipStream_t);
+    hipError_t (*bouzidi)(const lbk::BouzidiLinks&, void*, int, hipStream_t);
     hipError_t (*ghost_fill)(const lbk::GhostArgs&, hipStream_t);
-    hipError_t (*init_level)(float*, float*, float*, float*, float*, float*, int32_t, hipStream_t);
+    hipError_t (*init_level)(void*, void*, float*, float*, float*, float*, int32_t, int, hipStream_t);
     hipError_t (*init_thermal)(float*, float*, float*, float*, int32_t, float, hipStream_t);
     hipError_t (*forces)(const lbk::ForceArgs&, int, double*, hipStream_t);
-    hipError_t (*stats)(const float*, const float*, const float*, const uint8_t*, int32_t, double*, int, double*,
+    hipError_t (*stats)(const void*, const float*, const float*, const uint8_t*, int32_t, int, double*, int,
                         hipStream_t);
 };
 
@@ -61,7 +61,7 @@ struct Level {
     int32_t nb = 0;
     double dx = 0;
     float tau = 0, tau_g = 0;
-    float* f[2] = {nullptr, nullptr};
+    void* f[2] = {nullptr, nullptr};  // 4 or 2 bytes per population (lb_config.storage)
     float* rho[2] = {nullptr, nullptr};
     float* vel[2] = {nullptr, nullptr};
     float* g[2] = {nullptr, nullptr};
@@ -94,8 +94,8 @@ struct Level {
 
 struct Parent {
     const Level* P = nullptr;
-    const float *f_new = nullptr, *rho_new = nullptr, *vel_new = nullptr;
-    const float *f_old = nullptr, *rho_old = nullptr, *vel_old = nullptr;
+    const void *f_new = nullptr, *f_old = nullptr;
+    const float *rho_new = nullptr, *vel_new = nullptr, *rho_old = nullptr, *vel_old = nullptr;
     const float *g_new = nullptr, *T_new = nullptr, *g_old = nullptr, *T_old = nullptr;
 };
 
@@ -170,6 +170,67 @@ float half_to_float(uint16_t h) {
 }
 
 inline size_t fidx(int64_t b, int k, int c) { return ((size_t)b * Q + k) * NC + c; }
+
+// Population storage on the host side of transfers.  FP16/BF16 levels hold
+// f - w_k rounded to nearest even; the device decodes w_k + stored value.
+float wk_host(int k) {
+    const int cx = k % 3 - 1, cy = (k / 3) % 3 - 1, cz = k / 9 - 1;
+    const int d2 = cx * cx + cy * cy + cz * cz;
+    return d2 == 0 ? 8.0f / 27.0f : d2 == 1 ? 2.0f / 27.0f : d2 == 2 ? 1.0f / 54.0f : 1.0f / 216.0f;
+}
+
+size_t pop_bytes(int storage) { return storage == LB_STORAGE_FP32 ? 4 : 2; }
+
+uint16_t float_to_half(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+    if (ax > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);
+    if (ax >= 0x47800000u) return (uint16_t)(sign | 0x7c00u);  // |f| >= 65536 (and inf)
+    if (ax < 0x38800000u) {                                      // half subnormal range: multiples of 2^-24
+        const float v = std::nearbyint(std::fabs(f) * 16777216.0f);
+        return (uint16_t)(sign | (uint32_t)v);
+    }
+    uint32_t h = ((((ax >> 23) - 112u) << 10) | ((ax >> 13) & 0x3ffu));
+    const uint32_t rem = ax & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;  // carry into the exponent gives inf at 65520
+    return (uint16_t)(sign | h);
+}
+
+uint16_t float_to_bf16(float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((b >> 16) | 0x40u);
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+
+float bf16_to_float(uint16_t h) {
+    const uint32_t b = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+float decode_pop(const uint8_t* base, size_t i, int storage, float w) {
+    if (storage == LB_STORAGE_FP32) {
+        float f;
+        std::memcpy(&f, base + i * 4, 4);
+        return f;
+    }
+    uint16_t u;
+    std::memcpy(&u, base + i * 2, 2);
+    return w + (storage == LB_STORAGE_FP16 ? half_to_float(u) : bf16_to_float(u));
+}
+
+void encode_pop(uint8_t* base, size_t i, float f, int storage, float w) {
+    if (storage == LB_STORAGE_FP32) {
+        std::memcpy(base + i * 4, &f, 4);
+        return;
+    }
+    const uint16_t u = storage == LB_STORAGE_FP16 ? float_to_half(f - w) : float_to_bf16(f - w);
+    std::memcpy(base + i * 2, &u, 2);
+}
 
 #define RC(expr)                \
     do {                        \
@@ -279,6 +340,7 @@ int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, floa
     const int out = 1 - par;
     const bool ddf = c->cfg.ddf_enabled && L.g[0] != nullptr;
     const bool has_children = li + 1 < (int)c->levels.size();
+    const int storage = c->cfg.storage;
     if (L.nb > 0) {
         lbk::CollideArgs a;
         std::memset(&a, 0, sizeof a);
@@ -314,28 +376,13 @@ int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, floa
         a.sponge_blend = p->sponge_blend_distributions;
         a.seed = (int32_t)(t_sub % 1000000);
         a.is_l1 = pv == nullptr ? 1 : 0;
-        a.tw = tw;
-        a.temporal = c->cfg.temporal_interpolation ? 1 : 0;
-        if (pv) {
-            a.pf_new = pv->f_new;
-            a.prho_new = pv->rho_new;
-            a.pvel_new = pv->vel_new;
-            a.pf_old = pv->f_old;
-            a.prho_old = pv->rho_old;
-            a.pvel_old = pv->vel_old;
-            a.pgrid = pv->P->grid;
-            a.pdx = pv->P->pd[0];
-            a.pdy = pv->P->pd[1];
-            a.pdz = pv->P->pd[2];
-            a.pnb = pv->P->nb;
-            a.tau_parent = pv->P->tau;
-        } else {
-            a.tau_parent = 0.5f;
-        }
+        a.storage = storage;
+        const int temporal = c->cfg.temporal_interpolation ? 1 : 0;
         if (pv && L.ngent > 0) {
             lbk::GhostArgs gh;
             std::memset(&gh, 0, sizeof gh);
             gh.f = L.f[par];
+            gh.storage = storage;
             gh.entries = L.gent;
             gh.n = L.ngent;
             gh.borg = L.borg;
@@ -346,7 +393,7 @@ int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, floa
             gh.tau_fine = L.tau;
             gh.tau_coarse = pv->P->tau;
             gh.tw = tw;
-            gh.temporal = a.temporal;
+            gh.temporal = temporal;
             gh.comp = a.comp;
             gh.thermal = 0;
             HIPCHK(c, c->k->ghost_fill(gh, c->stream));
@@ -365,7 +412,7 @@ int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, floa
         }
         HIPCHK(c, c->k->collide(a, c->stream));
         if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
-        if (L.nlinks > 0) HIPCHK(c, c->k->bouzidi(links_of(L), L.f[out], c->stream));
+        if (L.nlinks > 0) HIPCHK(c, c->k->bouzidi(links_of(L), L.f[out], storage, c->stream));
         if (ddf) {
             lbk::ThermalArgs t;
             std::memset(&t, 0, sizeof t);
@@ -389,28 +436,11 @@ int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, floa
             t.adiabatic = p->wall_bc_adiabatic;
             t.sponge_blend = p->sponge_blend_distributions;
             t.is_l1 = a.is_l1;
-            t.tw = tw;
-            t.temporal = a.temporal;
-            if (pv) {
-                t.pg_new = pv->g_new;
-                t.pT_new = pv->T_new;
-                t.pvel_new = pv->vel_new;
-                t.pg_old = pv->g_old;
-                t.pT_old = pv->T_old;
-                t.pvel_old = pv->vel_old;
-                t.pgrid = pv->P->grid;
-                t.pdx = pv->P->pd[0];
-                t.pdy = pv->P->pd[1];
-                t.pdz = pv->P->pd[2];
-                t.pnb = pv->P->nb;
-                t.tau_g_parent = pv->P->tau_g;
-            } else {
-                t.tau_g_parent = 0.5f;
-            }
             if (pv && L.ngent > 0) {
                 lbk::GhostArgs gh;
                 std::memset(&gh, 0, sizeof gh);
                 gh.f = L.g[par];
+                gh.storage = LB_STORAGE_FP32;  // thermal populations are always FP32
                 gh.entries = L.gent;
                 gh.n = L.ngent;
                 gh.borg = L.borg;
@@ -421,12 +451,12 @@ int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, floa
                 gh.tau_fine = L.tau_g;
                 gh.tau_coarse = pv->P->tau_g;
                 gh.tw = tw;
-                gh.temporal = a.temporal;
+                gh.temporal = temporal;
                 gh.thermal = 1;
                 HIPCHK(c, c->k->ghost_fill(gh, c->stream));
             }
             HIPCHK(c, c->k->thermal(t, c->stream));
-            if (L.nlinks > 0) HIPCHK(c, c->k->bouzidi(links_of(L), L.g[out], c->stream));
+            if (L.nlinks > 0) HIPCHK(c, c->k->bouzidi(links_of(L), L.g[out], LB_STORAGE_FP32, c->stream));
         }
     }
     L.cur = out;
@@ -471,7 +501,8 @@ int lb_create(lb_ctx** out, int device, const lb_config* cfg) {
     *out = nullptr;
     if (cfg->n_levels < 1 || cfg->n_levels > 16 || cfg->domain_nx < 8 || cfg->domain_ny < 8 || cfg->domain_nz < 8)
         return LB_EINVAL;
-    if (cfg->storage != LB_STORAGE_FP32) return LB_EINVAL;
+    if (cfg->storage != LB_STORAGE_FP32 && cfg->storage != LB_STORAGE_FP16 && cfg->storage != LB_STORAGE_BF16)
+        return LB_EINVAL;
     lb_ctx* c = new lb_ctx();
     c->device = device;
     c->cfg = *cfg;
@@ -586,7 +617,9 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     L.ngent = (int64_t)gent.size() / 2;
     const size_t n4 = (size_t)nb * NC, n5 = ((size_t)nb + ngh) * NC * Q;
     for (int i = 0; i < 2; i++) {
-        RC(dalloc(c, &L.f[i], n5));
+        uint8_t* fb = nullptr;
+        RC(dalloc(c, &fb, n5 * pop_bytes(c->cfg.storage)));
+        L.f[i] = fb;
         RC(dalloc(c, &L.rho[i], n4));
         RC(dalloc(c, &L.vel[i], n4 * 3));
         if (c->cfg.ddf_enabled) {
@@ -645,7 +678,8 @@ int lb_init_equilibrium(lb_ctx* c, float T0) {
     RC(check_ctx(c));
     HIPCHK(c, hipSetDevice(c->device));
     for (auto& L : c->levels) {
-        HIPCHK(c, c->k->init_level(L.f[0], L.f[1], L.rho[0], L.rho[1], L.vel[0], L.vel[1], L.nb, c->stream));
+        HIPCHK(c, c->k->init_level(L.f[0], L.f[1], L.rho[0], L.rho[1], L.vel[0], L.vel[1], L.nb, c->cfg.storage,
+                                   c->stream));
         if (c->cfg.ddf_enabled) HIPCHK(c, c->k->init_thermal(L.g[0], L.g[1], L.T[0], L.T[1], L.nb, T0, c->stream));
         L.cur = 0;
     }
@@ -774,13 +808,17 @@ int lb_download_level(lb_ctx* c, int level_id, int field, float* dst) {
     switch (field) {
         case LB_FIELD_F:
         case LB_FIELD_G: {
-            const float* src = field == LB_FIELD_F ? L.f[L.cur] : L.g[L.cur];
+            const void* src = field == LB_FIELD_F ? L.f[L.cur] : (const void*)L.g[L.cur];
             if (!src) return fail(c, LB_EINVAL, "field not allocated");
-            std::vector<float> tmp(n4 * Q);
-            HIPCHK(c, hipMemcpy(tmp.data(), src, tmp.size() * 4, hipMemcpyDeviceToHost));
-            for (int64_t b = 0; b < L.nb; b++)
-                for (int k = 0; k < Q; k++)
-                    std::memcpy(dst + (size_t)k * n4 + (size_t)b * NC, tmp.data() + fidx(b, k, 0), NC * 4);
+            const int st = field == LB_FIELD_F ? c->cfg.storage : LB_STORAGE_FP32;
+            std::vector<uint8_t> tmp(n4 * Q * pop_bytes(st));
+            HIPCHK(c, hipMemcpy(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost));
+            for (int k = 0; k < Q; k++) {
+                const float w = wk_host(k);
+                for (int64_t b = 0; b < L.nb; b++)
+                    for (int cl = 0; cl < NC; cl++)
+                        dst[(size_t)k * n4 + (size_t)b * NC + cl] = decode_pop(tmp.data(), fidx(b, k, cl), st, w);
+            }
             return LB_OK;
         }
         case LB_FIELD_RHO:
@@ -817,15 +855,21 @@ int lb_upload_level(lb_ctx* c, int level_id, int field, const float* src) {
     size_t n;
     switch (field) {
         case LB_FIELD_F:
-        case LB_FIELD_G:
-            pair = field == LB_FIELD_F ? L.f : L.g;
-            if (!pair[0]) return fail(c, LB_EINVAL, "field not allocated");
-            tmp.resize(n4 * Q);
-            for (int64_t b = 0; b < L.nb; b++)
-                for (int k = 0; k < Q; k++)
-                    std::memcpy(tmp.data() + fidx(b, k, 0), src + (size_t)k * n4 + (size_t)b * NC, NC * 4);
-            n = tmp.size();
-            break;
+        case LB_FIELD_G: {
+            void* const dev[2] = {field == LB_FIELD_F ? L.f[0] : (void*)L.g[0],
+                                  field == LB_FIELD_F ? L.f[1] : (void*)L.g[1]};
+            if (!dev[0]) return fail(c, LB_EINVAL, "field not allocated");
+            const int st = field == LB_FIELD_F ? c->cfg.storage : LB_STORAGE_FP32;
+            std::vector<uint8_t> raw(n4 * Q * pop_bytes(st));
+            for (int k = 0; k < Q; k++) {
+                const float w = wk_host(k);
+                for (int64_t b = 0; b < L.nb; b++)
+                    for (int cl = 0; cl < NC; cl++)
+                        encode_pop(raw.data(), fidx(b, k, cl), src[(size_t)k * n4 + (size_t)b * NC + cl], st, w);
+            }
+            for (int i = 0; i < 2; i++) HIPCHK(c, hipMemcpy(dev[i], raw.data(), raw.size(), hipMemcpyHostToDevice));
+            return LB_OK;
+        }
         case LB_FIELD_RHO:
         case LB_FIELD_T:
             pair = field == LB_FIELD_RHO ? L.rho : L.T;
@@ -860,7 +904,8 @@ int lb_stats(lb_ctx* c, int level_id, double* out) {
     }
     for (int i = 0; i < 8; i++) out[i] = 0;
     if (L.nb == 0) return LB_OK;
-    HIPCHK(c, c->k->stats(L.f[L.cur], L.rho[L.cur], L.vel[L.cur], L.obst, L.nb, c->spart, groups, nullptr, c->stream));
+    HIPCHK(c, c->k->stats(L.f[L.cur], L.rho[L.cur], L.vel[L.cur], L.obst, L.nb, c->cfg.storage, c->spart, groups,
+                          c->stream));
     std::vector<double> part((size_t)groups * 7);
     HIPCHK(c, hipMemcpyAsync(part.data(), c->spart, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

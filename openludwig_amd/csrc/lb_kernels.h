@@ -21,8 +21,9 @@ constexpr int Q = 27;
 //   borg[b][3]   global 1-based cell coordinate of local (0,0,0) minus 1
 //   grid         dense block pointer (max active coords), 0-based ids, -1 absent
 struct CollideArgs {
-    const float* f_in;
-    float* f_out;
+    const void* f_in;        // populations, storage `storage` (0 FP32, 1 FP16 offset, 2 BF16 offset)
+    void* f_out;
+    int32_t storage;
     const float* vel_in;
     float* vel_out;
     float* rho_out;
@@ -40,20 +41,7 @@ struct CollideArgs {
     float omega_bulk, omega3, omega4, lambda;
     int32_t adaptive, limiter, comp, wall_model, sponge_blend;
     int32_t seed;
-    // parent (level > 1)
     int32_t is_l1;
-    const float* pf_new;
-    const float* prho_new;
-    const float* pvel_new;
-    const float* pf_old;
-    const float* prho_old;
-    const float* pvel_old;
-    const int32_t* pgrid;
-    int32_t pdx, pdy, pdz;
-    int32_t pnb;
-    float tau_parent;
-    float tw;
-    int32_t temporal;
 };
 
 struct ThermalArgs {
@@ -72,18 +60,6 @@ struct ThermalArgs {
     float u_inlet, T_inlet, T_wall;
     int32_t adiabatic, sponge_blend;
     int32_t is_l1;
-    const float* pg_new;
-    const float* pT_new;
-    const float* pvel_new;
-    const float* pg_old;
-    const float* pT_old;
-    const float* pvel_old;
-    const int32_t* pgrid;
-    int32_t pdx, pdy, pdz;
-    int32_t pnb;
-    float tau_g_parent;
-    float tw;
-    int32_t temporal;
 };
 
 // Bouzidi link list: corrected slot dst = a*f[s1] + b*f[s2], all indices
@@ -104,14 +80,15 @@ struct BouzidiLinks {
 // only, so it is evaluated once per (ghost cell, direction) into the ghost
 // block's slots before the level's stream-collide.
 struct GhostArgs {
-    float* f;                  // populations being pulled (ghost blocks at index >= nb)
+    void* f;                   // populations being pulled (ghost blocks at index >= nb)
+    int32_t storage;
     const int32_t* entries;    // [n][2]: (ghost block << 9 | cell, 27-bit direction mask)
     int64_t n;
     const int32_t* borg;       // block origins, ghost blocks included
-    const float* a_new;
+    const void* a_new;
     const float* s_new;
     const float* u_new;
-    const float* a_old;
+    const void* a_old;
     const float* s_old;
     const float* u_old;
     const int32_t* pgrid;
@@ -156,15 +133,15 @@ struct ForceArgs {
     namespace NS {                                                                                       \
     hipError_t collide(const lbk::CollideArgs& a, hipStream_t s);                                        \
     hipError_t thermal(const lbk::ThermalArgs& a, hipStream_t s);                                        \
-    hipError_t bouzidi(const lbk::BouzidiLinks& l, float* f, hipStream_t s);                             \
+    hipError_t bouzidi(const lbk::BouzidiLinks& l, void* f, int storage, hipStream_t s);                 \
     hipError_t ghost_fill(const lbk::GhostArgs& g, hipStream_t s);                                       \
-    hipError_t init_level(float* f0, float* f1, float* rho0, float* rho1, float* v0, float* v1, int32_t nb, \
-                          hipStream_t s);                                                                \
+    hipError_t init_level(void* f0, void* f1, float* rho0, float* rho1, float* v0, float* v1, int32_t nb,  \
+                          int storage, hipStream_t s);                                                   \
     hipError_t init_thermal(float* g0, float* g1, float* T0, float* T1, int32_t nb, float Tinit,         \
                             hipStream_t s);                                                              \
     hipError_t forces(const lbk::ForceArgs& a, int n_groups, double* out9, hipStream_t s);               \
-    hipError_t stats(const float* f, const float* rho, const float* vel, const uint8_t* obst, int32_t nb, \
-                     double* partials, int n_groups, double* out, hipStream_t s);                        \
+    hipError_t stats(const void* f, const float* rho, const float* vel, const uint8_t* obst, int32_t nb,  \
+                     int storage, double* partials, int n_groups, hipStream_t s);                        \
     }
 
 LBK_DECLARE(lbk_strict)

@@ -16,6 +16,7 @@
 //
 // This file is compiled twice; LBK_NS names the namespace (lbk_strict /
 // lbk_fast).
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
@@ -53,6 +54,46 @@ typedef __attribute__((address_space(1))) uint8_t GU8;
 __device__ __forceinline__ const GF* gp(const float* p) { return (const GF*)p; }
 __device__ __forceinline__ GF* gp(float* p) { return (GF*)p; }
 __device__ __forceinline__ const GU8* gp(const uint8_t* p) { return (const GU8*)p; }
+typedef __attribute__((address_space(1))) uint16_t GU16;
+
+// Population storage.  ST = 0: FP32.  ST = 1 / 2: the offset f - w_k in
+// binary16 / bfloat16, round-to-nearest-even; all arithmetic stays FP32 and a
+// stored value decodes to the float w_k + offset.
+template <int ST>
+struct Codec;
+template <>
+struct Codec<0> {
+    static __device__ __forceinline__ float load(const void* base, size_t i, float) {
+        return ((const GF*)base)[i];
+    }
+    static __device__ __forceinline__ void store(void* base, size_t i, float f, float) { ((GF*)base)[i] = f; }
+};
+template <>
+struct Codec<1> {
+    static __device__ __forceinline__ float load(const void* base, size_t i, float w) {
+        const uint16_t u = ((const GU16*)base)[i];
+        return w + __half2float(__ushort_as_half(u));
+    }
+    static __device__ __forceinline__ void store(void* base, size_t i, float f, float w) {
+        ((GU16*)base)[i] = __half_as_ushort(__float2half_rn(f - w));
+    }
+};
+template <>
+struct Codec<2> {
+    static __device__ __forceinline__ float load(const void* base, size_t i, float w) {
+        const uint32_t u = ((const GU16*)base)[i];
+        return w + __uint_as_float(u << 16);
+    }
+    static __device__ __forceinline__ uint16_t enc(float x) {
+        uint32_t b = __float_as_uint(x);
+        if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((b >> 16) | 0x40u);   // quiet NaN
+        b += 0x7fffu + ((b >> 16) & 1u);
+        return (uint16_t)(b >> 16);
+    }
+    static __device__ __forceinline__ void store(void* base, size_t i, float f, float w) {
+        ((GU16*)base)[i] = enc(f - w);
+    }
+};
 
 // Equilibria (src/collision/physics_utils.jl:19-82)
 __device__ __forceinline__ float feq2(float rho, float ux, float uy, float uz, float w, float cx, float cy, float cz) {
@@ -96,92 +137,15 @@ __device__ __forceinline__ float wk_rt(int k) {
     return d2 == 0 ? 8.0f / 27.0f : d2 == 1 ? 2.0f / 27.0f : d2 == 2 ? 1.0f / 54.0f : 1.0f / 216.0f;
 }
 
-// interpolate_with_rescaling (physics_interpolation.jl:4-127) and its thermal
-// twin (:135-248).  Parent cell P sits at coordinate P; lower corner clamped
-// to >= 1 after the weights; missing corners take corner 000's tuple.
-__device__ __forceinline__ float interp_rescale(const float* __restrict__ a_new, const float* __restrict__ s_new,
-                                             const float* __restrict__ u_new, const float* __restrict__ a_old,
-                                             const float* __restrict__ s_old, const float* __restrict__ u_old,
-                                             const int32_t* __restrict__ pgrid, int pdx, int pdy, int pdz,
-                                             int fgx, int fgy, int fgz, int k, float tau_fine, float tau_coarse,
-                                             float tw, int temporal, int comp, int thermal) {
-    float w_k = wk_rt(k), cx = (float)CX(k), cy = (float)CY(k), cz = (float)CZ(k);
-    float pxc = ((float)fgx - 0.5f) * 0.5f;
-    float pyc = ((float)fgy - 0.5f) * 0.5f;
-    float pzc = ((float)fgz - 0.5f) * 0.5f;
-    int px0 = (int)floorf(pxc), py0 = (int)floorf(pyc), pz0 = (int)floorf(pzc);
-    int px1 = px0 + 1, py1 = py0 + 1, pz1 = pz0 + 1;
-    float wx = pxc - (float)px0, wy = pyc - (float)py0, wz = pzc - (float)pz0;
-    px0 = max(1, px0);
-    py0 = max(1, py0);
-    pz0 = max(1, pz0);
-    float v[8][5];
-    bool use_old = temporal == 1 && tw < 0.99f;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        int pgx = (i & 1) ? px1 : px0;
-        int pgy = (i & 2) ? py1 : py0;
-        int pgz = (i & 4) ? pz1 : pz0;
-        int pbx = (pgx - 1) / BS + 1, pby = (pgy - 1) / BS + 1, pbz = (pgz - 1) / BS + 1;
-        bool ok = false;
-        if (pbx >= 1 && pbx <= pdx && pby >= 1 && pby <= pdy && pbz >= 1 && pbz <= pdz) {
-            int pb = pgrid[(pbx - 1) + pdx * ((pby - 1) + pdy * (pbz - 1))];
-            if (pb >= 0) {
-                int c = ((pgx - 1) % BS) + BS * ((pgy - 1) % BS) + BS * BS * ((pgz - 1) % BS);
-                size_t i4 = (size_t)pb * NC + c;
-                size_t iv = (size_t)pb * 3 * NC + c;
-                float fn = a_new[fidx(pb, k, c)], rn = s_new[i4];
-                float uxn = u_new[iv], uyn = u_new[iv + NC], uzn = u_new[iv + 2 * NC];
-                if (use_old) {
-                    float fo = a_old[fidx(pb, k, c)], ro = s_old[i4];
-                    float uxo = u_old[iv], uyo = u_old[iv + NC], uzo = u_old[iv + 2 * NC];
-                    v[i][0] = fo * (1.0f - tw) + fn * tw;
-                    v[i][1] = ro * (1.0f - tw) + rn * tw;
-                    v[i][2] = uxo * (1.0f - tw) + uxn * tw;
-                    v[i][3] = uyo * (1.0f - tw) + uyn * tw;
-                    v[i][4] = uzo * (1.0f - tw) + uzn * tw;
-                } else {
-                    v[i][0] = fn; v[i][1] = rn; v[i][2] = uxn; v[i][3] = uyn; v[i][4] = uzn;
-                }
-                ok = true;
-            }
-        }
-        if (!ok) {
-            if (i == 0) {
-                v[0][0] = w_k; v[0][1] = 1.0f; v[0][2] = 0.0f; v[0][3] = 0.0f; v[0][4] = 0.0f;
-            } else {
-#pragma unroll
-                for (int j = 0; j < 5; j++) v[i][j] = v[0][j];
-            }
-        }
-    }
-    float r[5];
-#pragma unroll
-    for (int j = 0; j < 5; j++) {
-        float c00 = v[0][j] * (1.0f - wx) + v[1][j] * wx;
-        float c01 = v[4][j] * (1.0f - wx) + v[5][j] * wx;
-        float c10 = v[2][j] * (1.0f - wx) + v[3][j] * wx;
-        float c11 = v[6][j] * (1.0f - wx) + v[7][j] * wx;
-        float c0 = c00 * (1.0f - wy) + c10 * wy;
-        float c1 = c01 * (1.0f - wy) + c11 * wy;
-        r[j] = c0 * (1.0f - wz) + c1 * wz;
-    }
-    float eq = thermal ? geq(r[1], r[2], r[3], r[4], w_k, cx, cy, cz)
-                       : feq_auto(r[1], r[2], r[3], r[4], w_k, cx, cy, cz, comp);
-    float neq = r[0] - eq;
-    float tc = tau_coarse - 0.5f;
-    float tf = tau_fine - 0.5f;
-    float scale = 1.0f;
-    if (tc > 1.0e-6f) {
-        scale = tf / tc;
-        scale = scale > 100.0f ? 100.0f : (scale < 0.01f ? 0.01f : scale);
-    }
-    return eq + neq * scale;
-}
-
-// Missing-neighbour cascade of stream_collide_cumulant! (kernel_cumulant.jl:117-159).
+// Missing-neighbour cascade of stream_collide_cumulant! (kernel_cumulant.jl:117-159)
+// for sources outside the domain.  An in-domain source without a block (the
+// coarse-to-fine case of :143-156) never reaches here: it is a ghost block
+// filled by ghost_kernel.  The last branch (a level-1 source inside the
+// domain with no block) cannot occur either, level 1 being the full box; it
+// returns w_k as the reference does.
+template <int ST>
 __device__ __forceinline__ float bc_value(const CollideArgs& a, int k, int b, int c, int sgx, int sgy, int sgz, int gy,
-                                       int gz) {
+                                          int gz) {
     float cx = (float)CX(k), cy = (float)CY(k), cz = (float)CZ(k), w = wk_rt(k);
     if (sgx < 1) {
         float noise = a.inlet_turb > 0.0f ? gradient_noise(gy, gz, a.seed, 1234) * a.inlet_turb * a.u_inlet : 0.0f;
@@ -189,11 +153,9 @@ __device__ __forceinline__ float bc_value(const CollideArgs& a, int k, int b, in
         return feq_auto(1.0f, u_inst, 0.0f, 0.0f, w, cx, cy, cz, a.comp);
     }
     if (sgx > a.nxg) return feq_auto(1.0f, a.u_inlet, 0.0f, 0.0f, w, cx, cy, cz, a.comp);
-    if (sgy < 1 || sgy > a.nyg) return a.f_in[fidx(b, MIRY(k), c)];
-    if (sgz < 1 || sgz > a.nzg) return a.f_in[fidx(b, MIRZ(k), c)];
-    if (!a.is_l1)
-        return interp_rescale(a.pf_new, a.prho_new, a.pvel_new, a.pf_old, a.prho_old, a.pvel_old, a.pgrid, a.pdx,
-                              a.pdy, a.pdz, sgx, sgy, sgz, k, a.tau, a.tau_parent, a.tw, a.temporal, a.comp, 0);
+    // the mirrored direction has the same |c|, hence the same weight
+    if (sgy < 1 || sgy > a.nyg) return Codec<ST>::load(a.f_in, fidx(b, MIRY(k), c), w);
+    if (sgz < 1 || sgz > a.nzg) return Codec<ST>::load(a.f_in, fidx(b, MIRZ(k), c), w);
     return w;
 }
 
@@ -235,6 +197,7 @@ __device__ __forceinline__ bool schedule(const int32_t* __restrict__ order, int 
 // stream_collide_cumulant! (src/collision/kernel_cumulant.jl:29-627), with the
 // A/B buffers chosen by the caller and no f_post_collision copy (Bouzidi
 // reads f_out before it corrects it, see bouzidi_gather below).
+template <int ST>
 __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
     __shared__ int snbr[27];
     __shared__ float sfix[27 * 256];
@@ -254,7 +217,6 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
     // before any is used: a missing neighbour block only swaps in the cell's
     // own slot as a harmless address, and the boundary cascade then replaces
     // the value (miss mask), keeping the loads free of branches.
-    const GF* __restrict__ fin = gp(a.f_in);
     float A[27];
     uint32_t miss = 0u;
 #pragma unroll
@@ -267,7 +229,7 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
         const int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
         const int blk = nbb >= 0 ? nbb : b;
         const int cc = nbb >= 0 ? ((sx & 7) + 8 * (sy & 7) + 64 * (sz & 7)) : c;
-        A[k] = fin[fidx(blk, k, cc)];
+        A[k] = Codec<ST>::load(a.f_in, fidx(blk, k, cc), WK(k));
         miss |= (nbb < 0 ? 1u : 0u) << k;
     }
     // WALE stencil of the previous velocity (physics_utils.jl:112-123), loaded
@@ -303,7 +265,7 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
 #pragma unroll 1
         for (int k = 0; k < 27; k++)
             if (miss & (1u << k))
-                sfix[k * 256 + threadIdx.x] = bc_value(a, k, b, c, gx - CX(k), gy - CY(k), gz - CZ(k), gy, gz);
+                sfix[k * 256 + threadIdx.x] = bc_value<ST>(a, k, b, c, gx - CX(k), gy - CY(k), gz - CZ(k), gy, gz);
 #pragma unroll
         for (int k = 0; k < 27; k++)
             if (miss & (1u << k)) A[k] = sfix[k * 256 + threadIdx.x];
@@ -322,7 +284,7 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
         if (cz == -1) jz += -val;
     }
 
-    GF* __restrict__ fo = gp(a.f_out);
+    void* const fo = a.f_out;
     GF* __restrict__ vo = gp(a.vel_out);
     GF* __restrict__ ro = gp(a.rho_out);
     if (is_obs) {
@@ -333,7 +295,7 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
         vo[iv + 2 * NC] = 0.0f;
         ro[i4] = 1.0f;
 #pragma unroll
-        for (int k = 0; k < 27; k++) fo[fidx(b, k, c)] = A[OPP(k)];
+        for (int k = 0; k < 27; k++) Codec<ST>::store(fo, fidx(b, k, c), A[OPP(k)], WK(k));
         return;
     }
 
@@ -596,11 +558,11 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
             float ft = WK(k) * 3.0f *
                        ((cxf - ux + 3.0f * cu * cxf) * Fx + (cyf - uy + 3.0f * cu * cyf) * Fy +
                         (czf - uz + 3.0f * cu * czf) * Fz);
-            fo[fidx(b, k, c)] = A[k] + pre * ft;
+            Codec<ST>::store(fo, fidx(b, k, c), A[k] + pre * ft, WK(k));
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < 27; k++) fo[fidx(b, k, c)] = A[k];
+        for (int k = 0; k < 27; k++) Codec<ST>::store(fo, fidx(b, k, c), A[k], WK(k));
     }
 }
 
@@ -608,7 +570,9 @@ inline unsigned padded_groups(int nb) { return ((unsigned)nb * 2u + 7u) & ~7u; }
 
 hipError_t collide(const CollideArgs& a, hipStream_t s) {
     if (a.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(collide_kernel, dim3(padded_groups(a.nb)), dim3(256), 0, s, a);
+    if (a.storage == 1) hipLaunchKernelGGL(collide_kernel<1>, dim3(padded_groups(a.nb)), dim3(256), 0, s, a);
+    else if (a.storage == 2) hipLaunchKernelGGL(collide_kernel<2>, dim3(padded_groups(a.nb)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(collide_kernel<0>, dim3(padded_groups(a.nb)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -620,9 +584,7 @@ __device__ __forceinline__ float thermal_bc(const ThermalArgs& a, int k, int b, 
     if (sgx > a.nxg) return geq(a.T_inlet, a.u_inlet, 0.0f, 0.0f, w, cx, cy, cz);
     if (sgy < 1 || sgy > a.nyg) return a.g_in[fidx(b, MIRY(k), c)];
     if (sgz < 1 || sgz > a.nzg) return a.g_in[fidx(b, MIRZ(k), c)];
-    if (!a.is_l1)
-        return interp_rescale(a.pg_new, a.pT_new, a.pvel_new, a.pg_old, a.pT_old, a.pvel_old, a.pgrid, a.pdx, a.pdy,
-                              a.pdz, sgx, sgy, sgz, k, a.tau_g, a.tau_g_parent, a.tw, a.temporal, 0, 1);
+    // in-domain sources without a block are ghost blocks (thermal ghost fill)
     return w * a.T_inlet;
 }
 
@@ -718,8 +680,9 @@ hipError_t thermal(const ThermalArgs& a, hipStream_t s) {
 // (physics_interpolation.jl:4-127, 135-248) for every direction a ghost cell
 // serves.  The corner geometry and the density/velocity part of the
 // trilinear blend depend on the cell only and are computed once; each
-// direction then blends its own population component.  Operation order per
-// value is that of interp_rescale above.
+// direction then blends its own population component.  Per value, the
+// operations are those of the reference function, in its order.
+template <int ST>
 __global__ void __launch_bounds__(256) ghost_kernel(const GhostArgs a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -798,7 +761,7 @@ __global__ void __launch_bounds__(256) ghost_kernel(const GhostArgs a) {
         scale = tf / tc;
         scale = scale > 100.0f ? 100.0f : (scale < 0.01f ? 0.01f : scale);
     }
-    GF* __restrict__ fo = gp(a.f);
+    void* const fo = a.f;
 #pragma unroll 1
     for (int k = 0; k < 27; k++) {
         if (!(mask & (1u << k))) continue;
@@ -807,9 +770,9 @@ __global__ void __launch_bounds__(256) ghost_kernel(const GhostArgs a) {
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             if (ok[j]) {
-                const float fn = a.a_new[fidx(cb[j], k, cc[j])];
+                const float fn = Codec<ST>::load(a.a_new, fidx(cb[j], k, cc[j]), w_k);
                 if (use_old) {
-                    const float fo_ = a.a_old[fidx(cb[j], k, cc[j])];
+                    const float fo_ = Codec<ST>::load(a.a_old, fidx(cb[j], k, cc[j]), w_k);
                     fv[j] = fo_ * (1.0f - tw) + fn * tw;
                 } else {
                     fv[j] = fn;
@@ -828,13 +791,16 @@ __global__ void __launch_bounds__(256) ghost_kernel(const GhostArgs a) {
         const float eq = a.thermal ? geq(r[0], r[1], r[2], r[3], w_k, cx, cy, cz)
                                    : feq_auto(r[0], r[1], r[2], r[3], w_k, cx, cy, cz, a.comp);
         const float neq = r0 - eq;
-        fo[fidx(gb, k, c)] = eq + neq * scale;
+        Codec<ST>::store(fo, fidx(gb, k, c), eq + neq * scale, w_k);
     }
 }
 
 hipError_t ghost_fill(const GhostArgs& g, hipStream_t s) {
     if (g.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ghost_kernel, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, s, g);
+    const dim3 grid((unsigned)((g.n + 255) / 256));
+    if (g.storage == 1) hipLaunchKernelGGL(ghost_kernel<1>, grid, dim3(256), 0, s, g);
+    else if (g.storage == 2) hipLaunchKernelGGL(ghost_kernel<2>, grid, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL(ghost_kernel<0>, grid, dim3(256), 0, s, g);
     return hipGetLastError();
 }
 
@@ -845,35 +811,50 @@ hipError_t ghost_fill(const GhostArgs& g, hipStream_t s) {
 // distinct, so the result equals the reference's read-from-f_post_collision
 // form without its full-size copy.  __fmul_rn/__fadd_rn keep the two
 // products and the sum rounded separately in both builds.
-__global__ void bouzidi_gather(const BouzidiLinks l, const float* __restrict__ f) {
+// All three slots of a link have the same |c|, so one weight w_k (from the
+// destination's direction) decodes and encodes them.
+template <int ST>
+__global__ void bouzidi_gather(const BouzidiLinks l, const void* __restrict__ f) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= l.n) return;
-    l.tmp[i] = __fadd_rn(__fmul_rn(l.a[i], f[l.s1[i]]), __fmul_rn(l.b[i], f[l.s2[i]]));
+    const float w = wk_rt((int)((l.dst[i] >> 9) % 27));
+    const float f1 = Codec<ST>::load(f, (size_t)l.s1[i], w), f2 = Codec<ST>::load(f, (size_t)l.s2[i], w);
+    l.tmp[i] = __fadd_rn(__fmul_rn(l.a[i], f1), __fmul_rn(l.b[i], f2));
 }
-__global__ void bouzidi_scatter(const BouzidiLinks l, float* __restrict__ f) {
+template <int ST>
+__global__ void bouzidi_scatter(const BouzidiLinks l, void* __restrict__ f) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= l.n) return;
-    f[l.dst[i]] = l.tmp[i];
+    const float w = wk_rt((int)((l.dst[i] >> 9) % 27));
+    Codec<ST>::store(f, (size_t)l.dst[i], l.tmp[i], w);
 }
 
-hipError_t bouzidi(const BouzidiLinks& l, float* f, hipStream_t s) {
-    if (l.n <= 0) return hipSuccess;
+template <int ST>
+void bouzidi_launch(const BouzidiLinks& l, void* f, hipStream_t s) {
     unsigned g = (unsigned)((l.n + 255) / 256);
-    hipLaunchKernelGGL(bouzidi_gather, dim3(g), dim3(256), 0, s, l, (const float*)f);
-    hipLaunchKernelGGL(bouzidi_scatter, dim3(g), dim3(256), 0, s, l, f);
+    hipLaunchKernelGGL(bouzidi_gather<ST>, dim3(g), dim3(256), 0, s, l, (const void*)f);
+    hipLaunchKernelGGL(bouzidi_scatter<ST>, dim3(g), dim3(256), 0, s, l, f);
+}
+
+hipError_t bouzidi(const BouzidiLinks& l, void* f, int storage, hipStream_t s) {
+    if (l.n <= 0) return hipSuccess;
+    if (storage == 1) bouzidi_launch<1>(l, f, s);
+    else if (storage == 2) bouzidi_launch<2>(l, f, s);
+    else bouzidi_launch<0>(l, f, s);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // init_eq! (main.jl:113-128): f = f_temp = w, rho = 1, vel = 0 on both halves
 // of every ping-pong pair (the old-state aliases included).
-__global__ void init_kernel(float* f0, float* f1, float* rho0, float* rho1, float* v0, float* v1) {
+template <int ST>
+__global__ void init_kernel(void* f0, void* f1, float* rho0, float* rho1, float* v0, float* v1) {
     const int b = blockIdx.x >> 1;
     const int c = ((blockIdx.x & 1) << 8) | threadIdx.x;
 #pragma unroll
     for (int k = 0; k < 27; k++) {
-        f0[fidx(b, k, c)] = WK(k);
-        f1[fidx(b, k, c)] = WK(k);
+        Codec<ST>::store(f0, fidx(b, k, c), WK(k), WK(k));
+        Codec<ST>::store(f1, fidx(b, k, c), WK(k), WK(k));
     }
     rho0[(size_t)b * NC + c] = 1.0f;
     rho1[(size_t)b * NC + c] = 1.0f;
@@ -883,10 +864,13 @@ __global__ void init_kernel(float* f0, float* f1, float* rho0, float* rho1, floa
         v1[iv + d * NC] = 0.0f;
     }
 }
-hipError_t init_level(float* f0, float* f1, float* rho0, float* rho1, float* v0, float* v1, int32_t nb,
+hipError_t init_level(void* f0, void* f1, float* rho0, float* rho1, float* v0, float* v1, int32_t nb, int storage,
                       hipStream_t s) {
     if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(init_kernel, dim3((unsigned)nb * 2), dim3(256), 0, s, f0, f1, rho0, rho1, v0, v1);
+    const dim3 grid((unsigned)nb * 2);
+    if (storage == 1) hipLaunchKernelGGL(init_kernel<1>, grid, dim3(256), 0, s, f0, f1, rho0, rho1, v0, v1);
+    else if (storage == 2) hipLaunchKernelGGL(init_kernel<2>, grid, dim3(256), 0, s, f0, f1, rho0, rho1, v0, v1);
+    else hipLaunchKernelGGL(init_kernel<0>, grid, dim3(256), 0, s, f0, f1, rho0, rho1, v0, v1);
     return hipGetLastError();
 }
 
@@ -1077,7 +1061,8 @@ hipError_t forces(const ForceArgs& a, int n_groups, double* out9, hipStream_t s)
 // ---------------------------------------------------------------------------
 // FP64 diagnostics per block: sum rho, KE, min/max rho over fluid cells,
 // non-finite populations, fluid count, sum of populations.
-__global__ void __launch_bounds__(256) stats_kernel(const float* __restrict__ f, const float* __restrict__ rho,
+template <int ST>
+__global__ void __launch_bounds__(256) stats_kernel(const void* __restrict__ f, const float* __restrict__ rho,
                                                     const float* __restrict__ vel, const uint8_t* __restrict__ obst,
                                                     int32_t nb, double* partials) {
     __shared__ double red[7][256];
@@ -1087,7 +1072,7 @@ __global__ void __launch_bounds__(256) stats_kernel(const float* __restrict__ f,
             size_t i4 = (size_t)blk * NC + c;
             double cell_f = 0;
             for (int k = 0; k < 27; k++) {
-                float v = f[fidx((int)blk, k, c)];
+                float v = Codec<ST>::load(f, fidx((int)blk, k, c), wk_rt(k));
                 if (!isfinite(v)) nonfin += 1;
                 cell_f += (double)v;
             }
@@ -1122,11 +1107,13 @@ __global__ void __launch_bounds__(256) stats_kernel(const float* __restrict__ f,
     if (threadIdx.x < 7) partials[(size_t)blockIdx.x * 7 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-hipError_t stats(const float* f, const float* rho, const float* vel, const uint8_t* obst, int32_t nb,
-                 double* partials, int n_groups, double* out, hipStream_t s) {
-    (void)out;
+hipError_t stats(const void* f, const float* rho, const float* vel, const uint8_t* obst, int32_t nb, int storage,
+                 double* partials, int n_groups, hipStream_t s) {
     if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(stats_kernel, dim3((unsigned)n_groups), dim3(256), 0, s, f, rho, vel, obst, nb, partials);
+    const dim3 grid((unsigned)n_groups);
+    if (storage == 1) hipLaunchKernelGGL(stats_kernel<1>, grid, dim3(256), 0, s, f, rho, vel, obst, nb, partials);
+    else if (storage == 2) hipLaunchKernelGGL(stats_kernel<2>, grid, dim3(256), 0, s, f, rho, vel, obst, nb, partials);
+    else hipLaunchKernelGGL(stats_kernel<0>, grid, dim3(256), 0, s, f, rho, vel, obst, nb, partials);
     return hipGetLastError();
 }
 

@@ -26,6 +26,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the calibration copy (0: none)")
     ap.add_argument("--strict", action="store_true")
+    ap.add_argument("--storage", type=int, default=0, help="population storage: 0 FP32, 1 FP16, 2 BF16 offsets")
+    ap.add_argument("--probe-gib", type=float, default=0.0, help="run the HBM bandwidth probe on this many GiB")
     args = ap.parse_args()
 
     from openludwig_amd import cases
@@ -36,7 +38,7 @@ def main() -> int:
     else:
         cfg, mesh = cases.sphere_re1e3()
     grids, params, mesh = build_case(cfg, mesh)
-    s = Solver(cfg, grids, params, mesh, strict_fp=args.strict)
+    s = Solver(cfg, grids, params, mesh, strict_fp=args.strict, storage=args.storage)
     s.init_equilibrium()
     t = 1
     for n in (args.warmup, args.steps):
@@ -46,6 +48,17 @@ def main() -> int:
     s.synchronize()
     print("levels", [(g.level_id, g.n_blocks, g.n_cells) for g in grids], flush=True)
     print("collide grid sizes (work-items):", [g.n_blocks * 512 for g in grids], flush=True)
+    if args.probe_gib > 0:
+        import ctypes as C
+
+        import numpy as np
+
+        from openludwig_amd import _native as N
+
+        out = np.zeros(3)
+        rc = N.hip_lib().lb_bandwidth_probe(0, int(args.probe_gib * (1 << 30)), 10, N.ptr(out, C.c_double))
+        print(f"bandwidth probe rc={rc}: copy {out[0]:.0f} GB/s, read {out[1]:.0f} GB/s, write {out[2]:.0f} GB/s",
+              flush=True)
     if args.copy_mib > 0:
         # one device-to-device copy through the HIP runtime (a blit kernel)
         import ctypes as C

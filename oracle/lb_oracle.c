@@ -159,6 +159,7 @@ typedef struct {
     int has_thermal;
     real *g, *g_temp, *g_post, *g_old, *T, *T_old;
     /* Bouzidi (sparse cell list, q per listed cell as FP16 bits)          */
+    int storage; /* population storage emulated (0 FP32, 1 FP16, 2 BF16)  */
     int nbc;
     int* cb;
     int8_t *cxl, *cyl, *czl;
@@ -169,6 +170,7 @@ typedef struct {
     int n_levels;
     orc_level lv[16];
     int domain_nx, domain_ny, domain_nz;
+    int storage;
 } orc_ctx;
 
 typedef struct {
@@ -204,6 +206,60 @@ static real half_to_real(uint16_t h) {
     else if (e == 31) v = m ? NAN : INFINITY;
     else v = ldexp((double)(m | 1024), e - 25);
     return (real)(s ? -v : v);
+}
+
+/* Population storage of the HIP library (include/ludwig_hip.h LB_STORAGE_*):
+ * FP16 / BF16 levels hold f - w_k rounded to nearest even and decode it as
+ * w_k + stored, all in FP32.  Every value written to population storage is
+ * replaced by what a later read returns.  Float -> binary16 and -> bfloat16
+ * follow IEEE 754 roundTiesToEven on the bit patterns.                      */
+static float round_half(float d) {
+    uint32_t x;
+    memcpy(&x, &d, 4);
+    uint32_t sign = x & 0x80000000u, ax = x & 0x7fffffffu;
+    if (ax > 0x7f800000u) return d;
+    float r;
+    if (ax >= 0x47800000u) {
+        r = INFINITY; /* |d| >= 2^16 */
+    } else if (ax < 0x38800000u) {
+        /* binary16 subnormals: integer multiples of 2^-24 */
+        r = nearbyintf(fabsf(d) * 16777216.0f) / 16777216.0f;
+    } else {
+        uint32_t keep = ax & ~0x1fffu, rem = ax & 0x1fffu;
+        if (rem > 0x1000u || (rem == 0x1000u && (ax & 0x2000u))) keep += 0x2000u;
+        if (keep >= 0x47800000u) keep = 0x7f800000u; /* rounded past 65504 */
+        memcpy(&r, &keep, 4);
+    }
+    uint32_t rb;
+    memcpy(&rb, &r, 4);
+    rb |= sign;
+    memcpy(&r, &rb, 4);
+    return r;
+}
+
+static float round_bf16(float d) {
+    uint32_t b;
+    memcpy(&b, &d, 4);
+    if ((b & 0x7fffffffu) > 0x7f800000u) return d;
+    b += 0x7fffu + ((b >> 16) & 1u);
+    b &= 0xffff0000u;
+    float r;
+    memcpy(&r, &b, 4);
+    return r;
+}
+
+static inline real store_round(int storage, real f, int k) {
+    if (storage == 0) return f;
+    float w = (float)W[k];
+    float d = (float)f - w;
+    float q = storage == 1 ? round_half(d) : round_bf16(d);
+    return (real)(w + q);
+}
+
+/* Exported for the tests that pin the rounding against numpy / torch. */
+double orc_store_round(int storage, int k, double f) {
+    build_lattice();
+    return (double)store_round(storage, (real)f, k);
 }
 
 static void* xcalloc(size_t n, size_t sz) {
@@ -248,6 +304,7 @@ int orc_set_level(orc_ctx* c, int lvl, int nb, const int* map_x, const int* map_
     orc_level* L_ = &c->lv[lvl];
     free_level(L_);
     L_->level_id = lvl + 1;
+    L_->storage = c->storage;
     L_->nb = nb;
     size_t n4 = (size_t)512 * nb, n5 = n4 * 27;
     L_->map_x = (int*)xcalloc(nb, sizeof(int)); L_->map_y = (int*)xcalloc(nb, sizeof(int));
@@ -521,7 +578,9 @@ static void collide_cell(const orc_level* L_, const orc_parent* pp, const orc_pa
                 } else if (zmin || zmax) {
                     val = f_in[i4 + n4 * MZ[k]];
                 } else if (!pp->is_l1) {
-                    val = interp_rescale(pp, 0, sgx, sgy, sgz, k, L_->tau, tw, prm->temporal, comp);
+                    /* the HIP library stores interpolated values in ghost blocks */
+                    val = store_round(L_->storage,
+                                      interp_rescale(pp, 0, sgx, sgy, sgz, k, L_->tau, tw, prm->temporal, comp), k);
                 } else {
                     val = W[k];
                 }
@@ -538,7 +597,7 @@ static void collide_cell(const orc_level* L_, const orc_parent* pp, const orc_pa
         vel_out[i4] = 0; vel_out[i4 + n4] = 0; vel_out[i4 + 2 * n4] = 0;
         L_->rho[i4] = L(1.0);
         for (int k = 0; k < 27; k++) {
-            real fc = fs[OPP[k]];
+            real fc = store_round(L_->storage, fs[OPP[k]], k);
             f_out[i4 + n4 * k] = fc;
             if (store_post) L_->f_post[i4 + n4 * k] = fc;
         }
@@ -776,14 +835,14 @@ static void collide_cell(const orc_level* L_, const orc_parent* pp, const orc_pa
         real ft = W[k] * L(3.0) *
                   ((cxf - ux + L(3.0) * cu * cxf) * Fx + (cyf - uy + L(3.0) * cu * cyf) * Fy +
                    (czf - uz + L(3.0) * cu * czf) * Fz);
-        real fc = A[k] + (L(1.0) - L(0.5) * om1) * ft;
+        real fc = store_round(L_->storage, A[k] + (L(1.0) - L(0.5) * om1) * ft, k);
         if (store_post) L_->f_post[i4 + n4 * k] = fc;
         f_out[i4 + n4 * k] = fc;
     }
 }
 
 /* bouzidi_correction_kernel_fixed!: src/geometry/bouzidi_kernel.jl:13-92 */
-static void bouzidi_apply(const orc_level* L_, real* f_out, const real* f_post, real q_min) {
+static void bouzidi_apply(const orc_level* L_, real* f_out, const real* f_post, real q_min, int storage) {
     size_t n4 = (size_t)512 * L_->nb;
     for (int i = 0; i < L_->nbc; i++) {
         int b = L_->cb[i], x = L_->cxl[i], y = L_->cyl[i], z = L_->czl[i];
@@ -812,12 +871,12 @@ static void bouzidi_apply(const orc_level* L_, real* f_out, const real* f_post, 
                         }
                     }
                     real c1 = L(2.0) * q;
-                    f_out[i4 + n4 * ok] = c1 * fk + (L(1.0) - c1) * fff;
+                    f_out[i4 + n4 * ok] = store_round(storage, c1 * fk + (L(1.0) - c1) * fff, ok);
                 } else {
                     real fop = f_post[i4 + n4 * ok];
                     real inv2q = L(1.0) / (L(2.0) * q);
                     real c2 = (L(2.0) * q - L(1.0)) * inv2q;
-                    f_out[i4 + n4 * ok] = inv2q * fk + c2 * fop;
+                    f_out[i4 + n4 * ok] = store_round(storage, inv2q * fk + c2 * fop, ok);
                 }
             }
         }
@@ -921,7 +980,7 @@ static void timestep_level(orc_ctx* c, orc_level* L_, const orc_parent* pp, cons
                 for (int x = 1; x <= BS; x++)
                     collide_cell(L_, pp, prm, f_out, f_in, vel_out, vel_in, store_post, u_curr, nxg, nyg, nzg, seed,
                                  tw, x, y, z, b);
-    if (L_->nbc > 0) bouzidi_apply(L_, f_out, L_->f_post, (real)prm->q_min);
+    if (L_->nbc > 0) bouzidi_apply(L_, f_out, L_->f_post, (real)prm->q_min, L_->storage);
     if (prm->ddf && g_out && g_in && L_->has_thermal) {
 #pragma omp parallel for schedule(static)
         for (int b = 1; b <= L_->nb; b++)
@@ -930,7 +989,7 @@ static void timestep_level(orc_ctx* c, orc_level* L_, const orc_parent* pp, cons
                     for (int x = 1; x <= BS; x++)
                         thermal_cell(L_, pp, prm, g_out, g_in, vel_out, store_post, u_curr, nxg, nyg, nzg, tw, x, y,
                                      z, b);
-        if (L_->nbc > 0) bouzidi_apply(L_, g_out, L_->g_post, (real)prm->q_min);
+        if (L_->nbc > 0) bouzidi_apply(L_, g_out, L_->g_post, (real)prm->q_min, 0);
     }
 }
 
@@ -1060,7 +1119,16 @@ int orc_put(orc_ctx* c, int lvl, int which, int t_last, const double* in) {
         case 4: if (!L_->has_thermal) return -2; dst = L_->T; n = n4; break;
         default: return -3;
     }
-    for (size_t i = 0; i < n; i++) dst[i] = (real)in[i];
+    for (size_t i = 0; i < n; i++)
+        dst[i] = which == 0 ? store_round(L_->storage, (real)in[i], (int)(i / n4)) : (real)in[i];
+    return 0;
+}
+
+/* Emulate the HIP library's population storage on every level. */
+int orc_set_storage(orc_ctx* c, int storage) {
+    if (!c || storage < 0 || storage > 2) return -1;
+    c->storage = storage;
+    for (int l = 0; l < c->n_levels; l++) c->lv[l].storage = storage;
     return 0;
 }
 

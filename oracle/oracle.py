@@ -66,6 +66,9 @@ def lib(precision: str = "f32") -> C.CDLL:
         L.orc_interp_probe.restype = C.c_double
         L.orc_interp_probe.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
                                        C.c_float, C.c_int, C.c_int]
+        L.orc_set_storage.argtypes = [vp, C.c_int]
+        L.orc_store_round.argtypes = [C.c_int, C.c_int, C.c_double]
+        L.orc_store_round.restype = C.c_double
         L.orc_gradient_noise.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         L.orc_gradient_noise.restype = C.c_float if precision == "f32" else C.c_double
         L.orc_lattice.argtypes = [ip, ip, ip, fp_ if precision == "f32" else dp, ip, ip, ip]
@@ -101,7 +104,9 @@ def _p(a, t):
 class Oracle:
     """Multilevel CPU restatement holding the same host tables as the GPU."""
 
-    def __init__(self, cfg, grids, params, precision: str = "f32"):
+    def __init__(self, cfg, grids, params, precision: str = "f32", storage: int = 0):
+        """storage: population storage of the HIP library to emulate
+        (0 FP32, 1 FP16 offset, 2 BF16 offset)."""
         self.L = lib(precision)
         self.cfg = cfg
         self.grids = grids
@@ -133,6 +138,8 @@ class Oracle:
                                       _p(cy, C.c_int8), _p(cz, C.c_int8), _p(q, C.c_uint16))
             if rc != 0:
                 raise RuntimeError(f"orc_set_level failed for level {i + 1}")
+        if self.L.orc_set_storage(self.ctx, int(storage)) != 0:
+            raise RuntimeError("orc_set_storage failed")
 
     def close(self):
         if self.ctx:

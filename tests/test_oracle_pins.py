@@ -316,3 +316,56 @@ def test_ramp_velocity():
         assert a == np.float32(b)
         ref = 0.05 * (0.5 * (1 - math.cos(math.pi * min(be, 500) / 500)))
         assert abs(a - ref) < 1e-8
+
+
+def _storage_cases():
+    """f values whose offsets f - w cover normals, binary16 subnormals, exact
+    ties of both formats, overflow and signed zero."""
+    rng = np.random.default_rng(5)
+    w = np.float32(8) / np.float32(27)
+    d = np.concatenate([
+        rng.uniform(-0.3, 0.3, 4000), rng.uniform(-1e-4, 1e-4, 2000),   # normals and binary16 subnormals
+        rng.uniform(-6e-8, 6e-8, 500),                                  # below half the smallest subnormal
+        [0.0, -0.0, 6.5e4, 6.6e4, -7e4, 65519.0, 65520.0],             # overflow threshold of binary16
+    ]).astype(np.float32)
+    # exact ties: midpoints between neighbouring binary16 values
+    h = rng.integers(0, 0x7bff, 500).astype(np.uint16)
+    lo = h.view(np.float16).astype(np.float32)
+    hi = (h + 1).astype(np.uint16).view(np.float16).astype(np.float32)
+    d = np.concatenate([d, (lo + hi) / 2, -(lo + hi) / 2]).astype(np.float32)
+    # exact bfloat16 ties: low 16 bits = 0x8000
+    b = (rng.integers(0x3000, 0x4000, 500).astype(np.uint32) << 16) | 0x8000
+    d = np.concatenate([d, b.view(np.float32)]).astype(np.float32)
+    return w, d
+
+
+def test_storage_round_fp16_matches_numpy():
+    """Offset storage FP16: w + binary16(f - w), RNE; numpy's float32 ->
+    float16 conversion is IEEE roundTiesToEven and is the check."""
+    L = lib("f32")
+    w, d = _storage_cases()
+    f = (w + d).astype(np.float32)
+    got = np.array([L.orc_store_round(1, 13, float(x)) for x in f])
+    dd = (f - w).astype(np.float32)
+    with np.errstate(over="ignore"):
+        exp = (w + dd.astype(np.float16).astype(np.float32)).astype(np.float32)
+    assert np.array_equal(got.astype(np.float32), exp, equal_nan=True)
+
+
+def test_storage_round_bf16_matches_torch():
+    """Offset storage BF16: torch's float32 -> bfloat16 cast (RNE) is the
+    check."""
+    torch = pytest.importorskip("torch")
+    L = lib("f32")
+    w, d = _storage_cases()
+    f = (w + d).astype(np.float32)
+    got = np.array([L.orc_store_round(2, 13, float(x)) for x in f]).astype(np.float32)
+    dd = torch.from_numpy((f - w).astype(np.float32))
+    exp = (w + dd.to(torch.bfloat16).to(torch.float32).numpy()).astype(np.float32)
+    assert np.array_equal(got, exp, equal_nan=True)
+
+
+def test_storage_round_fp32_is_identity():
+    L = lib("f32")
+    for x in (0.1, 0.3, -1.5, 1e-30):
+        assert L.orc_store_round(0, 0, x) == float(np.float32(x))
