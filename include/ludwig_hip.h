@@ -192,6 +192,14 @@ int lb_stats(lb_ctx* ctx, int level_id, double* out);
 /* Device bytes the context holds (serving admission).                    */
 int64_t lb_device_bytes(const lb_ctx* ctx);
 
+/* Device bytes a case would hold, computed before anything is allocated
+ * (no GPU needed): what lb_create, lb_level_upload of every level and
+ * lb_mesh_upload of n_tri triangles allocate, plus the engine's ramp table
+ * of n_ramp_batches values (lb_engine_add).  Used to admit serving requests
+ * against the HBM budget (no counterpart in the reference).                */
+int lb_footprint(const lb_config* cfg, const lb_level_desc* levels, int32_t n_levels, int64_t n_tri,
+                 int32_t n_ramp_batches, int64_t* bytes);
+
 /* Kernel timing (no counterpart in the reference, which only times the host
  * loop, main.jl:232): when enabled, HIP events are recorded on the context's
  * stream around every stream-collide launch.  lb_profile_read synchronises,
@@ -199,6 +207,50 @@ int64_t lb_device_bytes(const lb_ctx* ctx);
  * last read, and resets them.                                              */
 int lb_profile_enable(lb_ctx* ctx, int enable);
 int lb_profile_read(lb_ctx* ctx, int level_id, double* ms, int64_t* launches);
+
+/* ------------------------------------------------------------------------
+ * Batched engine (no counterpart in the reference, whose run_all_cases,
+ * main.jl:425-469, runs one case after another).  Cases are built as usual
+ * (lb_create, lb_level_upload, lb_mesh_upload, lb_init_equilibrium) and then
+ * attached; one engine step advances every attached case together:
+ * the 2:1 recursion is run over depths, and each launch covers the same depth
+ * of all cases over a concatenated block list (cases with fewer levels are
+ * aligned on their finest level and advance 2^(Lmax - L) coarse steps per
+ * engine step).  The step is captured as a hipGraph and replayed.  Per case,
+ * the sequence of coarse steps, seeds and inlet velocities is that of
+ * execute_timestep_batch! with batches of `batch_len` steps, so a case's
+ * results do not depend on which other cases share the engine.
+ * While attached, a case may only be used through the engine.            */
+typedef struct lb_engine lb_engine;
+
+int lb_engine_create(lb_engine** e, int device, int32_t storage, int32_t strict_fp);
+void lb_engine_destroy(lb_engine* e);
+const char* lb_engine_last_error(const lb_engine* e);
+
+/* Attach an initialised case for coarse steps 1..n_coarse_steps.  u_batches
+ * holds the inlet velocity of each ramp batch of batch_len coarse steps
+ * (n_batches >= ceil(n_coarse_steps / batch_len)), as the host loop computes
+ * it (main.jl:214-218).  p: the physics parameters of lb_step_batch.       */
+int lb_engine_add(lb_engine* e, lb_ctx* ctx, int64_t n_coarse_steps, const float* u_batches, int32_t n_batches,
+                  int32_t batch_len, const lb_step_params* p);
+
+/* Detach a case (waits for the engine's work); its fields then hold the state
+ * after its last completed coarse step and lb_forces etc. apply again.     */
+int lb_engine_remove(lb_engine* e, lb_ctx* ctx);
+
+/* Enqueue n engine steps (asynchronous). */
+int lb_engine_step(lb_engine* e, int32_t n_steps);
+int lb_engine_synchronize(lb_engine* e);
+
+/* Coarse steps a case has completed once the enqueued work has run. */
+int lb_engine_case_progress(const lb_engine* e, const lb_ctx* ctx, int64_t* t_done);
+
+/* Launch options (picked by the autotuner; results do not depend on them):
+ * graph (1: replay a captured hipGraph per step) and the work-group
+ * schedule of the stream-collide launches (LB_SCHED_*).                    */
+#define LB_SCHED_XCD 0    /* work-groups dealt to XCDs in contiguous runs */
+#define LB_SCHED_LINEAR 1 /* work-group i takes half-tile i               */
+int lb_engine_set_options(lb_engine* e, int32_t use_graph, int32_t schedule);
 
 /* HBM bandwidth probe (no counterpart in the reference): streams `bytes`
  * with 16-byte accesses; gbps_out[0] copy (read + write counted),

@@ -83,7 +83,22 @@ HIP_EXPORTS = {
     "lb_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "lb_profile_read": (C.c_int, [C.c_void_p, C.c_int, f64p, i64p]),
     "lb_bandwidth_probe": (C.c_int, [C.c_int, C.c_int64, C.c_int, f64p]),
+    "lb_engine_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int32, C.c_int32]),
+    "lb_engine_destroy": (None, [C.c_void_p]),
+    "lb_engine_last_error": (C.c_char_p, [C.c_void_p]),
+    "lb_engine_add": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, f32p, C.c_int32, C.c_int32,
+                                C.POINTER(LbStepParams)]),
+    "lb_engine_remove": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lb_engine_step": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lb_engine_synchronize": (C.c_int, [C.c_void_p]),
+    "lb_engine_case_progress": (C.c_int, [C.c_void_p, C.c_void_p, i64p]),
+    "lb_engine_set_options": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "lb_footprint": (C.c_int, [C.POINTER(LbConfig), C.POINTER(LbLevelDesc), C.c_int32, C.c_int64, C.c_int32,
+                               i64p]),
 }
+
+LB_SCHED_XCD = 0
+LB_SCHED_LINEAR = 1
 
 SETUP_EXPORTS = {
     "lbs_last_error": (C.c_char_p, []),

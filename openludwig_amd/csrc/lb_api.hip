@@ -37,10 +37,16 @@ constexpr int NC = 512;
 constexpr int Q = 27;
 
 struct Kern {
-    hipError_t (*collide)(const lbk::CollideArgs&, hipStream_t);
-    hipError_t (*thermal)(const lbk::ThermalArgs&, hipStream_t);
-    hipError_t (*bouzidi)(const lbk::BouzidiLinks&, void*, int, hipStream_t);
-    hipError_t (*ghost_fill)(const lbk::GhostArgs&, hipStream_t);
+    hipError_t (*collide)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
+                          int, hipStream_t);
+    hipError_t (*thermal)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
+                          hipStream_t);
+    hipError_t (*ghost_fill)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
+                             hipStream_t);
+    hipError_t (*bouzidi)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
+                          hipStream_t);
+    hipError_t (*set_state)(lbk::CaseState*, const lbk::CaseState&, hipStream_t);
+    hipError_t (*advance)(lbk::CaseState*, int32_t, hipStream_t);
     hipError_t (*init_level)(void*, void*, float*, float*, float*, float*, int32_t, int, hipStream_t);
     hipError_t (*init_thermal)(float*, float*, float*, float*, int32_t, float, hipStream_t);
     hipError_t (*forces)(const lbk::ForceArgs&, int, double*, hipStream_t);
@@ -48,12 +54,11 @@ struct Kern {
                         hipStream_t);
 };
 
-const Kern kStrict = {lbk_strict::collide, lbk_strict::thermal, lbk_strict::bouzidi, lbk_strict::ghost_fill,
-                      lbk_strict::init_level,
-                      lbk_strict::init_thermal, lbk_strict::forces, lbk_strict::stats};
-const Kern kFast = {lbk_fast::collide, lbk_fast::thermal, lbk_fast::bouzidi, lbk_fast::ghost_fill,
-                    lbk_fast::init_level,
-                    lbk_fast::init_thermal, lbk_fast::forces, lbk_fast::stats};
+#define LBK_TABLE(NS)                                                                                     \
+    {NS::collide, NS::thermal, NS::ghost_fill, NS::bouzidi, NS::set_state, NS::advance, NS::init_level,  \
+     NS::init_thermal, NS::forces, NS::stats}
+const Kern kStrict = LBK_TABLE(lbk_strict);
+const Kern kFast = LBK_TABLE(lbk_fast);
 
 struct Level {
     bool uploaded = false;
@@ -86,17 +91,20 @@ struct Level {
     float* la = nullptr;
     float* lb = nullptr;
     float* ltmp = nullptr;
-    int cur = 0;  // buffer index written by the last step (0 before any step)
+    bool wide = false;    // population buffer >= 4 GiB: kernels use 64-bit offsets
+    bool wide_g = false;  // the same for the thermal populations
+    // work lists over this level alone (LevelArgs index = level index):
+    // blocks in processing order, ghost-entry chunks, Bouzidi-link chunks
+    lbk::Work* wcol = nullptr;
+    int32_t n_gchunks = 0;
+    lbk::Work* wgh = nullptr;
+    int32_t n_lchunks = 0;
+    lbk::Work* wlk = nullptr;
+    int cur = 0;        // buffer index written by the last step (0 before any step)
+    std::vector<int32_t> order_host;  // processing order (engine work lists)
     // kernel timing: event pairs recorded around collide launches
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     size_t ev_used = 0;
-};
-
-struct Parent {
-    const Level* P = nullptr;
-    const void *f_new = nullptr, *f_old = nullptr;
-    const float *rho_new = nullptr, *vel_new = nullptr, *rho_old = nullptr, *vel_old = nullptr;
-    const float *g_new = nullptr, *T_new = nullptr, *g_old = nullptr, *T_old = nullptr;
 };
 
 }  // namespace
@@ -118,6 +126,52 @@ struct lb_ctx {
     double* spart = nullptr;
     int n_stat_groups = 0;
     bool profile = false;
+    // device-resident kernel arguments: the case record and one LevelArgs per level
+    lbk::CaseState* st_dev = nullptr;
+    lbk::LevelArgs* las_dev = nullptr;
+    std::vector<lbk::LevelArgs> las_host;
+    lb_engine* engine = nullptr;  // set while attached to a batched engine
+    bool dry = false;             // footprint planning: nothing is allocated or copied
+};
+
+// One attached case of an engine.
+struct EngineCase {
+    lb_ctx* c = nullptr;
+    int64_t n_total = 0;
+    int64_t t0 = 1;          // next coarse step (host copy of the device record)
+    int32_t t_inc = 1;
+    int32_t batch = 8;
+    float* u_tab = nullptr;  // device, owned by the case context
+    lb_step_params p{};
+};
+
+// Work lists of one depth of the engine's recursion (concatenated over cases).
+struct DepthWork {
+    lbk::Work *col = nullptr, *gh = nullptr, *lk = nullptr, *tcol = nullptr, *tgh = nullptr, *tlk = nullptr;
+    int32_t ncol = 0, ngh = 0, nlk = 0, ntcol = 0, ntgh = 0, ntlk = 0;
+    int wide = 0, wide_g = 0;
+};
+
+struct lb_engine {
+    int device = 0;
+    int32_t storage = 0;
+    const Kern* k = &kStrict;
+    hipStream_t stream = nullptr;
+    std::vector<EngineCase> cases;
+    std::string err;
+    // batched launch data, rebuilt when the case set changes
+    bool dirty = true;
+    int lmax = 0;
+    std::vector<void*> allocs;
+    lbk::LevelArgs* las = nullptr;
+    lbk::CaseState* states = nullptr;  // one record per attached case, contiguous
+    std::vector<DepthWork> depth;
+    int32_t n_active = 0;
+    // options
+    bool use_graph = true;
+    int sched = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -142,6 +196,10 @@ template <class T>
 int dalloc(lb_ctx* c, T** p, size_t n) {
     *p = nullptr;
     size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    if (c->dry) {  // footprint planning: count only
+        c->bytes += (int64_t)bytes;
+        return LB_OK;
+    }
     hipError_t e = hipMalloc((void**)p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
@@ -155,6 +213,7 @@ int dalloc(lb_ctx* c, T** p, size_t n) {
 
 template <class T>
 int h2d(lb_ctx* c, T* dst, const T* src, size_t n) {
+    if (c->dry) return LB_OK;
     hipError_t e = hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(c, LB_EHIP, "hipMemcpy H2D: %s", hipGetErrorString(e));
     return LB_OK;
@@ -170,6 +229,16 @@ float half_to_float(uint16_t h) {
 }
 
 inline size_t fidx(int64_t b, int k, int c) { return ((size_t)b * Q + k) * NC + c; }
+
+lbk::Work mkwork(int li, int item, int ci, const int32_t* row) {
+    lbk::Work w;
+    w.li = li;
+    w.item = item;
+    w.ci = ci;
+    w.pad = 0;
+    w.nbr_row = row;
+    return w;
+}
 
 // Population storage on the host side of transfers.  FP16/BF16 levels hold
 // f - w_k rounded to nearest even; the device decodes w_k + stored value.
@@ -319,85 +388,19 @@ int build_links(lb_ctx* c, Level& L, const lb_level_desc* d) {
     return LB_OK;
 }
 
-lbk::BouzidiLinks links_of(const Level& L) {
-    lbk::BouzidiLinks l;
-    l.n = L.nlinks;
-    l.dst = L.ldst;
-    l.s1 = L.ls1;
-    l.s2 = L.ls2;
-    l.a = L.la;
-    l.b = L.lb;
-    l.tmp = L.ltmp;
-    return l;
-}
-
 // perform_timestep_v2! + recursive_step! for one level (timestep.jl:1-159,
-// solver_control.jl:6-197).
-int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, float tw, const lb_step_params* p) {
+// solver_control.jl:6-197): substep j of level li in the coarse step held by
+// the case record (t_sub = t0 * 2^li + j), then both child substeps.  The
+// kernels resolve buffers, parity, seed and temporal weight from (record, j).
+int step_level(lb_ctx* c, int li, int64_t j) {
     if (li >= (int)c->levels.size()) return LB_OK;
     Level& L = c->levels[(size_t)li];
-    const int par = (int)(t_sub & 1);
-    const int out = 1 - par;
     const bool ddf = c->cfg.ddf_enabled && L.g[0] != nullptr;
-    const bool has_children = li + 1 < (int)c->levels.size();
     const int storage = c->cfg.storage;
+    const lbk::LevelArgs* las = c->las_dev;
+    const lbk::CaseState* st = c->st_dev;
     if (L.nb > 0) {
-        lbk::CollideArgs a;
-        std::memset(&a, 0, sizeof a);
-        a.f_in = L.f[par];
-        a.f_out = L.f[out];
-        a.vel_in = L.vel[par];
-        a.vel_out = L.vel[out];
-        a.rho_out = L.rho[out];
-        a.obst = L.obst;
-        a.sponge = L.sponge;
-        a.wall_dist = L.wall_dist;
-        a.nbr = L.nbr;
-        a.borg = L.borg;
-        a.order = L.order;
-        a.nb = L.nb;
-        const int scale = 1 << (L.id - 1);
-        a.nxg = c->cfg.domain_nx * scale;
-        a.nyg = c->cfg.domain_ny * scale;
-        a.nzg = c->cfg.domain_nz * scale;
-        a.tau = L.tau;
-        a.u_inlet = u;
-        a.c_wale = p->c_wale;
-        a.nu_sgs_bg = p->nu_sgs_background;
-        a.inlet_turb = p->inlet_turbulence;
-        a.omega_bulk = p->omega_bulk;
-        a.omega3 = p->omega_3;
-        a.omega4 = p->omega_4;
-        a.lambda = p->lambda_param;
-        a.adaptive = p->adaptive_omega4;
-        a.limiter = p->limiter;
-        a.comp = p->compressible;
-        a.wall_model = (p->wall_model && L.wall_dist) ? 1 : 0;
-        a.sponge_blend = p->sponge_blend_distributions;
-        a.seed = (int32_t)(t_sub % 1000000);
-        a.is_l1 = pv == nullptr ? 1 : 0;
-        a.storage = storage;
-        const int temporal = c->cfg.temporal_interpolation ? 1 : 0;
-        if (pv && L.ngent > 0) {
-            lbk::GhostArgs gh;
-            std::memset(&gh, 0, sizeof gh);
-            gh.f = L.f[par];
-            gh.storage = storage;
-            gh.entries = L.gent;
-            gh.n = L.ngent;
-            gh.borg = L.borg;
-            gh.a_new = pv->f_new; gh.s_new = pv->rho_new; gh.u_new = pv->vel_new;
-            gh.a_old = pv->f_old; gh.s_old = pv->rho_old; gh.u_old = pv->vel_old;
-            gh.pgrid = pv->P->grid;
-            gh.pdx = pv->P->pd[0]; gh.pdy = pv->P->pd[1]; gh.pdz = pv->P->pd[2];
-            gh.tau_fine = L.tau;
-            gh.tau_coarse = pv->P->tau;
-            gh.tw = tw;
-            gh.temporal = temporal;
-            gh.comp = a.comp;
-            gh.thermal = 0;
-            HIPCHK(c, c->k->ghost_fill(gh, c->stream));
-        }
+        if (li > 0 && L.n_gchunks > 0) HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 0, c->stream));
         hipEvent_t e1 = nullptr;
         if (c->profile) {
             if (L.ev_used == L.ev.size()) {
@@ -410,79 +413,278 @@ int step_level(lb_ctx* c, int li, int64_t t_sub, const Parent* pv, float u, floa
             e1 = L.ev[L.ev_used].second;
             L.ev_used++;
         }
-        HIPCHK(c, c->k->collide(a, c->stream));
+        HIPCHK(c, c->k->collide(las, st, L.wcol, L.nb, j, storage, L.wide ? 1 : 0, 0, c->stream));
         if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
-        if (L.nlinks > 0) HIPCHK(c, c->k->bouzidi(links_of(L), L.f[out], storage, c->stream));
+        if (L.n_lchunks > 0) HIPCHK(c, c->k->bouzidi(las, st, L.wlk, L.n_lchunks, j, storage, 0, c->stream));
         if (ddf) {
-            lbk::ThermalArgs t;
-            std::memset(&t, 0, sizeof t);
-            t.g_in = L.g[par];
-            t.g_out = L.g[out];
-            t.vel = L.vel[out];
-            t.T_out = L.T[out];
-            t.obst = L.obst;
-            t.sponge = L.sponge;
-            t.nbr = L.nbr;
-            t.borg = L.borg;
-            t.order = L.order;
-            t.nb = L.nb;
-            t.nxg = a.nxg;
-            t.nyg = a.nyg;
-            t.nzg = a.nzg;
-            t.tau_g = L.tau_g;
-            t.u_inlet = u;
-            t.T_inlet = p->T_inlet;
-            t.T_wall = p->T_wall;
-            t.adiabatic = p->wall_bc_adiabatic;
-            t.sponge_blend = p->sponge_blend_distributions;
-            t.is_l1 = a.is_l1;
-            if (pv && L.ngent > 0) {
-                lbk::GhostArgs gh;
-                std::memset(&gh, 0, sizeof gh);
-                gh.f = L.g[par];
-                gh.storage = LB_STORAGE_FP32;  // thermal populations are always FP32
-                gh.entries = L.gent;
-                gh.n = L.ngent;
-                gh.borg = L.borg;
-                gh.a_new = pv->g_new; gh.s_new = pv->T_new; gh.u_new = pv->vel_new;
-                gh.a_old = pv->g_old; gh.s_old = pv->T_old; gh.u_old = pv->vel_old;
-                gh.pgrid = pv->P->grid;
-                gh.pdx = pv->P->pd[0]; gh.pdy = pv->P->pd[1]; gh.pdz = pv->P->pd[2];
-                gh.tau_fine = L.tau_g;
-                gh.tau_coarse = pv->P->tau_g;
-                gh.tw = tw;
-                gh.temporal = temporal;
-                gh.thermal = 1;
-                HIPCHK(c, c->k->ghost_fill(gh, c->stream));
-            }
-            HIPCHK(c, c->k->thermal(t, c->stream));
-            if (L.nlinks > 0) HIPCHK(c, c->k->bouzidi(links_of(L), L.g[out], LB_STORAGE_FP32, c->stream));
+            if (li > 0 && L.n_gchunks > 0)
+                HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 1, c->stream));
+            HIPCHK(c, c->k->thermal(las, st, L.wcol, L.nb, j, L.wide_g ? 1 : 0, 0, c->stream));
+            if (L.n_lchunks > 0) HIPCHK(c, c->k->bouzidi(las, st, L.wlk, L.n_lchunks, j, storage, 1, c->stream));
         }
     }
-    L.cur = out;
-    if (has_children) {
-        Parent ch;
-        ch.P = &L;
-        ch.f_new = L.f[out];
-        ch.rho_new = L.rho[out];
-        ch.vel_new = L.vel[out];
-        ch.f_old = L.f[par];
-        ch.rho_old = L.rho[par];
-        ch.vel_old = L.vel[par];
-        if (ddf) {
-            ch.g_new = L.g[out];
-            ch.T_new = L.T[out];
-            ch.g_old = L.g[par];
-            ch.T_old = L.T[par];
-        }
-        RC(step_level(c, li + 1, 2 * t_sub, &ch, u, 0.0f, p));
-        RC(step_level(c, li + 1, 2 * t_sub + 1, &ch, u, 0.5f, p));
-    }
+    RC(step_level(c, li + 1, 2 * j));
+    RC(step_level(c, li + 1, 2 * j + 1));
     return LB_OK;
+}
+
+// The case record for coarse step t with the given inlet velocity and the
+// physics parameters of one lb_step_batch call.
+lbk::CaseState case_record(int64_t t, float u, const lb_step_params* p, const lb_config& cfg) {
+    lbk::CaseState s;
+    std::memset(&s, 0, sizeof s);
+    s.t0 = t;
+    s.n_total = INT64_MAX;
+    s.u = u;
+    s.active = 1;
+    s.t_inc = 1;
+    s.batch = 1;
+    s.u_tab = nullptr;
+    s.c_wale = p->c_wale;
+    s.nu_sgs_bg = p->nu_sgs_background;
+    s.inlet_turb = p->inlet_turbulence;
+    s.omega_bulk = p->omega_bulk;
+    s.omega3 = p->omega_3;
+    s.omega4 = p->omega_4;
+    s.lambda = p->lambda_param;
+    s.adaptive = p->adaptive_omega4;
+    s.limiter = p->limiter;
+    s.comp = p->compressible;
+    s.wall_model = p->wall_model ? 1 : 0;
+    s.sponge_blend = p->sponge_blend_distributions;
+    s.T_inlet = p->T_inlet;
+    s.T_wall = p->T_wall;
+    s.adiabatic = p->wall_bc_adiabatic;
+    s.temporal = cfg.temporal_interpolation ? 1 : 0;
+    return s;
+}
+
+// Output half of every level after coarse step t: the last substep of level
+// li is t * 2^li + 2^li - 1.
+void set_current(lb_ctx* c, int64_t t) {
+    for (size_t li = 0; li < c->levels.size(); li++) {
+        const int64_t last = (t << li) + ((int64_t)1 << li) - 1;
+        c->levels[li].cur = 1 - (int)(last & 1);
+    }
+}
+
+// Work list of 256-item chunks of one level (ghost entries, Bouzidi links).
+int chunk_list(lb_ctx* c, int li, int64_t n, lbk::Work** out, int32_t* n_chunks) {
+    *n_chunks = (int32_t)((n + 255) / 256);
+    if (*n_chunks == 0) return LB_OK;
+    std::vector<lbk::Work> w((size_t)*n_chunks);
+    for (int32_t i = 0; i < *n_chunks; i++) w[(size_t)i] = mkwork(li, i * 256, 0, nullptr);
+    RC(dalloc(c, out, w.size()));
+    return h2d(c, *out, w.data(), w.size());
+}
+
+// LevelArgs of level li (host copy + device record); the parent's pairs come
+// from level li - 1, which is uploaded first.
+int upload_level_args(lb_ctx* c, int li) {
+    const Level& L = c->levels[(size_t)li];
+    lbk::LevelArgs& A = c->las_host[(size_t)li];
+    std::memset((void*)&A, 0, sizeof A);
+    for (int i = 0; i < 2; i++) {
+        A.f[i] = L.f[i];
+        A.rho[i] = L.rho[i];
+        A.vel[i] = L.vel[i];
+        A.g[i] = L.g[i];
+        A.T[i] = L.T[i];
+    }
+    A.obst = L.obst;
+    A.sponge = L.sponge;
+    A.wall_dist = L.wall_dist;
+    A.nbr = L.nbr;
+    A.borg = L.borg;
+    A.nb = L.nb;
+    const int scale = 1 << (L.id - 1);
+    A.nxg = c->cfg.domain_nx * scale;
+    A.nyg = c->cfg.domain_ny * scale;
+    A.nzg = c->cfg.domain_nz * scale;
+    A.lshift = L.id - 1;
+    A.tau = L.tau;
+    A.tau_g = L.tau_g;
+    if (li > 0) {
+        const Level& P = c->levels[(size_t)li - 1];
+        for (int i = 0; i < 2; i++) {
+            A.pf[i] = P.f[i];
+            A.prho[i] = P.rho[i];
+            A.pvel[i] = P.vel[i];
+            A.pg[i] = P.g[i];
+            A.pT[i] = P.T[i];
+        }
+        A.pgrid = P.grid;
+        A.pdx = P.pd[0];
+        A.pdy = P.pd[1];
+        A.pdz = P.pd[2];
+        A.ptau = P.tau;
+        A.ptau_g = P.tau_g;
+    }
+    A.gent = L.gent;
+    A.ngent = L.ngent;
+    A.links.n = L.nlinks;
+    A.links.dst = L.ldst;
+    A.links.s1 = L.ls1;
+    A.links.s2 = L.ls2;
+    A.links.a = L.la;
+    A.links.b = L.lb;
+    A.links.tmp = L.ltmp;
+    return h2d(c, c->las_dev + li, &A, 1);
+}
+
+int efail(lb_engine* e, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (e) e->err = buf;
+    return code;
+}
+
+#define EHIPCHK(eng, expr)                                                                                \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return efail(eng, LB_EHIP, "%s: %s", #expr, hipGetErrorString(e_));        \
+    } while (0)
+
+void engine_free_lists(lb_engine* e) {
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+    for (void* p : e->allocs) (void)hipFree(p);
+    e->allocs.clear();
+    e->las = nullptr;
+    e->states = nullptr;
+    e->depth.clear();
+}
+
+template <class T>
+int eupload(lb_engine* e, T** dst, const std::vector<T>& v) {
+    *dst = nullptr;
+    if (v.empty()) return LB_OK;
+    EHIPCHK(e, hipMalloc((void**)dst, v.size() * sizeof(T)));
+    e->allocs.push_back((void*)*dst);
+    EHIPCHK(e, hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return LB_OK;
+}
+
+// The device record of an attached case at its current position.
+lbk::CaseState engine_record(const lb_engine* e, const EngineCase& ec) {
+    lbk::CaseState r = case_record(ec.t0, 0.0f, &ec.p, ec.c->cfg);
+    r.n_total = ec.n_total;
+    r.active = ec.t0 <= ec.n_total ? 1 : 0;
+    r.t_inc = ec.t_inc;
+    r.batch = ec.batch;
+    r.u_tab = ec.u_tab;
+    (void)e;
+    return r;
+}
+
+// Rebuild the concatenated LevelArgs, the per-depth work lists and the case
+// records after the case set changed.  Cases are aligned on their finest
+// level: level l of a case with L levels runs at depth l + (Lmax - L).
+int engine_rebuild(lb_engine* e) {
+    EHIPCHK(e, hipStreamSynchronize(e->stream));
+    engine_free_lists(e);
+    e->lmax = 0;
+    for (const EngineCase& ec : e->cases) e->lmax = std::max(e->lmax, (int)ec.c->levels.size());
+    std::vector<lbk::LevelArgs> las;
+    std::vector<lbk::CaseState> sts;
+    std::vector<int> base;
+    for (EngineCase& ec : e->cases) {
+        base.push_back((int)las.size());
+        for (const lbk::LevelArgs& a : ec.c->las_host) las.push_back(a);
+        ec.t_inc = 1 << (e->lmax - (int)ec.c->levels.size());
+        sts.push_back(engine_record(e, ec));
+    }
+    RC(eupload(e, &e->las, las));
+    RC(eupload(e, &e->states, sts));
+    e->depth.assign((size_t)e->lmax, DepthWork());
+    for (int d = 1; d <= e->lmax; d++) {
+        std::vector<lbk::Work> col, gh, lk, tcol, tgh, tlk;
+        DepthWork& W = e->depth[(size_t)d - 1];
+        for (size_t i = 0; i < e->cases.size(); i++) {
+            const lb_ctx* c = e->cases[i].c;
+            const int L = (int)c->levels.size();
+            const int l = d - (e->lmax - L);
+            if (l < 1 || l > L) continue;
+            const Level& lev = c->levels[(size_t)l - 1];
+            const int li = base[i] + l - 1, ci = (int)i;
+            if (lev.nb == 0) continue;
+            const bool ddf = c->cfg.ddf_enabled && lev.g[0] != nullptr;
+            for (int32_t b : lev.order_host) {
+                const lbk::Work w = mkwork(li, b, ci, lev.nbr + (size_t)b * Q);
+                col.push_back(w);
+                if (ddf) tcol.push_back(w);
+            }
+            if (l > 1)
+                for (int64_t q = 0; q < lev.ngent; q += 256) {
+                    gh.push_back(mkwork(li, (int)q, ci, nullptr));
+                    if (ddf) tgh.push_back(mkwork(li, (int)q, ci, nullptr));
+                }
+            for (int64_t q = 0; q < lev.nlinks; q += 256) {
+                lk.push_back(mkwork(li, (int)q, ci, nullptr));
+                if (ddf) tlk.push_back(mkwork(li, (int)q, ci, nullptr));
+            }
+            if (lev.wide) W.wide = 1;
+            if (lev.wide_g) W.wide_g = 1;
+        }
+        RC(eupload(e, &W.col, col));
+        RC(eupload(e, &W.gh, gh));
+        RC(eupload(e, &W.lk, lk));
+        RC(eupload(e, &W.tcol, tcol));
+        RC(eupload(e, &W.tgh, tgh));
+        RC(eupload(e, &W.tlk, tlk));
+        W.ncol = (int32_t)col.size();
+        W.ngh = (int32_t)gh.size();
+        W.nlk = (int32_t)lk.size();
+        W.ntcol = (int32_t)tcol.size();
+        W.ntgh = (int32_t)tgh.size();
+        W.ntlk = (int32_t)tlk.size();
+    }
+    e->n_active = (int32_t)e->cases.size();
+    e->dirty = false;
+    return LB_OK;
+}
+
+// One depth-d operation (substep j) and its subtree, over all cases.
+hipError_t engine_op(lb_engine* e, int d, int64_t j) {
+    const DepthWork& W = e->depth[(size_t)d - 1];
+    const Kern* k = e->k;
+    hipError_t r = hipSuccess;
+#define ETRY(x)                       \
+    do {                              \
+        r = (x);                      \
+        if (r != hipSuccess) return r; \
+    } while (0)
+    const lbk::CaseState* st = e->states;
+    if (W.ngh) ETRY(k->ghost_fill(e->las, st, W.gh, W.ngh, j, e->storage, 0, e->stream));
+    if (W.ncol) ETRY(k->collide(e->las, st, W.col, W.ncol, j, e->storage, W.wide, e->sched, e->stream));
+    if (W.nlk) ETRY(k->bouzidi(e->las, st, W.lk, W.nlk, j, e->storage, 0, e->stream));
+    if (W.ntcol) {
+        if (W.ntgh) ETRY(k->ghost_fill(e->las, st, W.tgh, W.ntgh, j, e->storage, 1, e->stream));
+        ETRY(k->thermal(e->las, st, W.tcol, W.ntcol, j, W.wide_g, e->sched, e->stream));
+        if (W.ntlk) ETRY(k->bouzidi(e->las, st, W.tlk, W.ntlk, j, e->storage, 1, e->stream));
+    }
+    if (d < e->lmax) {
+        ETRY(engine_op(e, d + 1, 2 * j));
+        ETRY(engine_op(e, d + 1, 2 * j + 1));
+    }
+#undef ETRY
+    return r;
+}
+
+hipError_t engine_issue_step(lb_engine* e) {
+    hipError_t r = engine_op(e, 1, 0);
+    if (r != hipSuccess) return r;
+    return e->k->advance(e->states, (int32_t)e->cases.size(), e->stream);
 }
 
 int check_ctx(lb_ctx* c) {
     if (!c) return LB_EINVAL;
+    if (c->engine) return fail(c, LB_ESTATE, "case is attached to an engine");
     for (size_t i = 0; i < c->levels.size(); i++)
         if (!c->levels[i].uploaded) return fail(c, LB_ESTATE, "level %zu not uploaded", i + 1);
     return LB_OK;
@@ -518,6 +720,12 @@ int lb_create(lb_ctx** out, int device, const lb_config* cfg) {
         return LB_EHIP;
     }
     c->levels.resize((size_t)cfg->n_levels);
+    c->las_host.resize((size_t)cfg->n_levels);
+    std::memset((void*)c->las_host.data(), 0, c->las_host.size() * sizeof(lbk::LevelArgs));
+    if (dalloc(c, &c->st_dev, 1) != LB_OK || dalloc(c, &c->las_dev, (size_t)cfg->n_levels) != LB_OK) {
+        lb_destroy(c);
+        return LB_ENOMEM;
+    }
     *out = c;
     return LB_OK;
 }
@@ -550,7 +758,7 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
         return fail(c, LB_EINVAL, "level %d: missing arrays", d->level_id);
     if (d->ptr_dim_x < 0 || d->ptr_dim_y < 0 || d->ptr_dim_z < 0)
         return fail(c, LB_EINVAL, "level %d: bad block_pointer dims", d->level_id);
-    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->dry) HIPCHK(c, hipSetDevice(c->device));
     L.id = d->level_id;
     L.nb = nb;
     L.tau = d->tau;
@@ -616,14 +824,18 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     L.ngh = ngh;
     L.ngent = (int64_t)gent.size() / 2;
     const size_t n4 = (size_t)nb * NC, n5 = ((size_t)nb + ngh) * NC * Q;
+    L.wide = (n5 + 512) * pop_bytes(c->cfg.storage) >= ((size_t)1 << 32);
+    L.wide_g = (n5 + 512) * 4 >= ((size_t)1 << 32);
     for (int i = 0; i < 2; i++) {
+        // 512 slots of padding: pulls from a missing neighbour block read a
+        // harmless slot up to 200 past the cell's own block
         uint8_t* fb = nullptr;
-        RC(dalloc(c, &fb, n5 * pop_bytes(c->cfg.storage)));
+        RC(dalloc(c, &fb, (n5 + 512) * pop_bytes(c->cfg.storage)));
         L.f[i] = fb;
         RC(dalloc(c, &L.rho[i], n4));
         RC(dalloc(c, &L.vel[i], n4 * 3));
         if (c->cfg.ddf_enabled) {
-            RC(dalloc(c, &L.g[i], n5));
+            RC(dalloc(c, &L.g[i], n5 + 512));
             RC(dalloc(c, &L.T[i], n4));
         }
     }
@@ -656,6 +868,12 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
         std::vector<int32_t> order((size_t)nb);
         for (int32_t i = 0; i < nb; i++) order[(size_t)i] = key[(size_t)i].second;
         RC(h2d(c, L.order, order.data(), order.size()));
+        L.order_host = order;
+        std::vector<lbk::Work> wcol((size_t)nb);
+        for (int32_t i = 0; i < nb; i++)
+            wcol[(size_t)i] = mkwork(L.id - 1, order[(size_t)i], 0, L.nbr + (size_t)order[(size_t)i] * Q);
+        RC(dalloc(c, &L.wcol, wcol.size()));
+        RC(h2d(c, L.wcol, wcol.data(), wcol.size()));
         RC(h2d(c, L.obst, d->obstacle, n4));
         RC(h2d(c, L.sponge, d->sponge, n4));
         if (d->wall_dist) RC(h2d(c, L.wall_dist, d->wall_dist, n4));
@@ -670,6 +888,9 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
         RC(h2d(c, L.grid, grid.data(), ng));
     }
     RC(build_links(c, L, d));
+    RC(chunk_list(c, L.id - 1, L.ngent, &L.wgh, &L.n_gchunks));
+    RC(chunk_list(c, L.id - 1, L.nlinks, &L.wlk, &L.n_lchunks));
+    RC(upload_level_args(c, L.id - 1));
     L.uploaded = true;
     return LB_OK;
 }
@@ -691,7 +912,11 @@ int lb_step_batch(lb_ctx* c, int64_t t_start, int32_t n_steps, float u_curr, con
     RC(check_ctx(c));
     if (!p || n_steps < 0 || t_start < 0) return fail(c, LB_EINVAL, "bad step arguments");
     HIPCHK(c, hipSetDevice(c->device));
-    for (int64_t t = t_start; t < t_start + n_steps; t++) RC(step_level(c, 0, t, nullptr, u_curr, 0.0f, p));
+    for (int64_t t = t_start; t < t_start + n_steps; t++) {
+        HIPCHK(c, c->k->set_state(c->st_dev, case_record(t, u_curr, p, c->cfg), c->stream));
+        RC(step_level(c, 0, 0));
+        set_current(c, t);
+    }
     return LB_OK;
 }
 
@@ -704,9 +929,9 @@ int lb_synchronize(lb_ctx* c) {
 
 int lb_mesh_upload(lb_ctx* c, int64_t n, const float* cx, const float* cy, const float* cz, const float* nx,
                    const float* ny, const float* nz, const float* areas) {
-    if (!c || n < 0 || (n > 0 && (!cx || !cy || !cz || !nx || !ny || !nz || !areas))) return LB_EINVAL;
+    if (!c || n < 0 || (!c->dry && n > 0 && (!cx || !cy || !cz || !nx || !ny || !nz || !areas))) return LB_EINVAL;
     if (c->n_tri != 0) return fail(c, LB_ESTATE, "mesh already uploaded");
-    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->dry) HIPCHK(c, hipSetDevice(c->device));
     const float* src[7] = {cx, cy, cz, nx, ny, nz, areas};
     for (int i = 0; i < 7; i++) {
         RC(dalloc(c, &c->mesh[i], (size_t)n));
@@ -947,6 +1172,156 @@ int lb_profile_read(lb_ctx* c, int level_id, double* ms, int64_t* launches) {
     *ms = tot;
     *launches = (int64_t)L.ev_used;
     L.ev_used = 0;
+    return LB_OK;
+}
+
+int lb_engine_create(lb_engine** out, int device, int32_t storage, int32_t strict_fp) {
+    if (!out) return LB_EINVAL;
+    *out = nullptr;
+    if (storage < LB_STORAGE_FP32 || storage > LB_STORAGE_BF16) return LB_EINVAL;
+    lb_engine* e = new lb_engine();
+    e->device = device;
+    e->storage = storage;
+    e->k = strict_fp ? &kStrict : &kFast;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return LB_EHIP;
+    }
+    *out = e;
+    return LB_OK;
+}
+
+void lb_engine_destroy(lb_engine* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    for (EngineCase& ec : e->cases) ec.c->engine = nullptr;
+    engine_free_lists(e);
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+const char* lb_engine_last_error(const lb_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+int lb_engine_add(lb_engine* e, lb_ctx* c, int64_t n_coarse_steps, const float* u_batches, int32_t n_batches,
+                  int32_t batch_len, const lb_step_params* p) {
+    if (!e || !c || !u_batches || !p) return LB_EINVAL;
+    RC(check_ctx(c));
+    if (c->device != e->device) return efail(e, LB_EINVAL, "case is on device %d, engine on %d", c->device, e->device);
+    if (c->cfg.storage != e->storage) return efail(e, LB_EINVAL, "case storage differs from the engine's");
+    if ((c->k == &kStrict) != (e->k == &kStrict)) return efail(e, LB_EINVAL, "case and engine builds differ");
+    if (n_coarse_steps < 1 || batch_len < 1 || (int64_t)n_batches * batch_len < n_coarse_steps)
+        return efail(e, LB_EINVAL, "bad step count or ramp table");
+    EHIPCHK(e, hipSetDevice(e->device));
+    EHIPCHK(e, hipStreamSynchronize(c->stream));
+    EngineCase ec;
+    ec.c = c;
+    ec.n_total = n_coarse_steps;
+    ec.t0 = 1;
+    ec.batch = batch_len;
+    ec.p = *p;
+    if (dalloc(c, &ec.u_tab, (size_t)n_batches) != LB_OK) return efail(e, LB_ENOMEM, "%s", c->err.c_str());
+    if (h2d(c, ec.u_tab, u_batches, (size_t)n_batches) != LB_OK) return efail(e, LB_EHIP, "%s", c->err.c_str());
+    c->engine = e;
+    e->cases.push_back(ec);
+    e->dirty = true;
+    return LB_OK;
+}
+
+int lb_engine_remove(lb_engine* e, lb_ctx* c) {
+    if (!e || !c) return LB_EINVAL;
+    EHIPCHK(e, hipSetDevice(e->device));
+    for (size_t i = 0; i < e->cases.size(); i++) {
+        if (e->cases[i].c != c) continue;
+        EHIPCHK(e, hipStreamSynchronize(e->stream));
+        const int64_t done = std::min(e->cases[i].t0 - 1, e->cases[i].n_total);
+        if (done >= 1) set_current(c, done);
+        c->engine = nullptr;
+        e->cases.erase(e->cases.begin() + (long)i);
+        e->dirty = true;
+        return LB_OK;
+    }
+    return efail(e, LB_EINVAL, "case is not attached to this engine");
+}
+
+int lb_engine_step(lb_engine* e, int32_t n) {
+    if (!e || n < 0) return LB_EINVAL;
+    EHIPCHK(e, hipSetDevice(e->device));
+    if (e->cases.empty() || n == 0) return LB_OK;
+    if (e->dirty) RC(engine_rebuild(e));
+    if (e->use_graph && !e->exec) {
+        EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t r = engine_issue_step(e);
+        hipGraph_t g = nullptr;
+        const hipError_t r2 = hipStreamEndCapture(e->stream, &g);
+        if (r != hipSuccess || r2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return efail(e, LB_EHIP, "graph capture failed: %s", hipGetErrorString(r != hipSuccess ? r : r2));
+        }
+        e->graph = g;
+        EHIPCHK(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    }
+    for (int32_t i = 0; i < n; i++) {
+        if (e->use_graph) EHIPCHK(e, hipGraphLaunch(e->exec, e->stream));
+        else EHIPCHK(e, engine_issue_step(e));
+        for (EngineCase& ec : e->cases)
+            if (ec.t0 <= ec.n_total) ec.t0 += ec.t_inc;
+    }
+    return LB_OK;
+}
+
+int lb_engine_synchronize(lb_engine* e) {
+    if (!e) return LB_EINVAL;
+    EHIPCHK(e, hipSetDevice(e->device));
+    EHIPCHK(e, hipStreamSynchronize(e->stream));
+    return LB_OK;
+}
+
+int lb_engine_case_progress(const lb_engine* e, const lb_ctx* c, int64_t* t_done) {
+    if (!e || !c || !t_done) return LB_EINVAL;
+    for (const EngineCase& ec : e->cases)
+        if (ec.c == c) {
+            *t_done = std::min(ec.t0 - 1, ec.n_total);
+            return LB_OK;
+        }
+    return LB_EINVAL;
+}
+
+int lb_engine_set_options(lb_engine* e, int32_t use_graph, int32_t schedule) {
+    if (!e || (schedule != LB_SCHED_XCD && schedule != LB_SCHED_LINEAR)) return LB_EINVAL;
+    EHIPCHK(e, hipSetDevice(e->device));
+    EHIPCHK(e, hipStreamSynchronize(e->stream));
+    e->use_graph = use_graph != 0;
+    e->sched = schedule;
+    // the captured graph bakes the schedule in
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+    return LB_OK;
+}
+
+// Device bytes a case will hold: the allocations of lb_create,
+// lb_level_upload (every level) and lb_mesh_upload, counted by running the
+// same host code without allocating (no GPU needed), plus the engine's ramp
+// table of n_ramp_batches values.  lb_stats adds 57,344 bytes on first use.
+int lb_footprint(const lb_config* cfg, const lb_level_desc* levels, int32_t n_levels, int64_t n_tri,
+                 int32_t n_ramp_batches, int64_t* bytes) {
+    if (!cfg || !levels || !bytes || n_levels != cfg->n_levels || n_levels < 1 || n_levels > 16) return LB_EINVAL;
+    lb_ctx c;
+    c.dry = true;
+    c.cfg = *cfg;
+    c.levels.resize((size_t)n_levels);
+    c.las_host.resize((size_t)n_levels);
+    RC(dalloc(&c, &c.st_dev, 1));
+    RC(dalloc(&c, &c.las_dev, (size_t)n_levels));
+    for (int i = 0; i < n_levels; i++) RC(lb_level_upload(&c, &levels[i]));
+    if (n_tri > 0) RC(lb_mesh_upload(&c, n_tri, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+    if (n_ramp_batches > 0) {
+        float* u = nullptr;
+        RC(dalloc(&c, &u, (size_t)n_ramp_batches));
+    }
+    *bytes = c.bytes;
     return LB_OK;
 }
 

@@ -15,51 +15,35 @@ constexpr int NC = 512;  // cells per block
 constexpr int Q = 27;
 
 // Storage of one level on the device (block-major SoA):
-//   f[b][k][c]   c = x + 8y + 64z (0-based local cell), float
+//   f[b][k][c]   c = x + 8y + 64z (0-based local cell), FP32 or 16-bit offsets
 //   rho[b][c], vel[b][d][c], obst[b][c], sponge[b][c], wall_dist[b][c]
-//   nbr[b][27]   0-based neighbour block, -1 absent (dir as k)
+//   nbr[b][27]   0-based neighbour block, -1 absent (dir as k); ghost blocks
+//                (coarse-to-fine interface sources) have indices >= nb
 //   borg[b][3]   global 1-based cell coordinate of local (0,0,0) minus 1
 //   grid         dense block pointer (max active coords), 0-based ids, -1 absent
-struct CollideArgs {
-    const void* f_in;        // populations, storage `storage` (0 FP32, 1 FP16 offset, 2 BF16 offset)
-    void* f_out;
-    int32_t storage;
-    const float* vel_in;
-    float* vel_out;
-    float* rho_out;
-    const uint8_t* obst;
-    const float* sponge;
-    const float* wall_dist;  // may be null when wall_model == 0
-    const int32_t* nbr;
-    const int32_t* borg;
-    const int32_t* order;  // processing order (Morton-sorted block indices)
-    int32_t nb;
-    int32_t nxg, nyg, nzg;
-    float tau;
-    float u_inlet;
+//
+// Kernels take their arguments from device memory: a LevelArgs record per
+// (case, level) and a CaseState per case, named by a work list whose entries
+// are (LevelArgs index, item, CaseState index).  One launch can therefore cover one level of
+// one case or the same depth of many cases (batched serving); a hipGraph of
+// such launches replays without host-side argument changes.
+
+// Per-case state: the step position and the physics parameters of
+// lb_step_params.  Substep j of level l in the current step has
+// t_sub = t0 * 2^(l-1) + j (the recursion of solver_control.jl:6-197).
+struct CaseState {
+    int64_t t0;           // coarse step that the current step starts at
+    int64_t n_total;      // last coarse step; substeps of later coarse steps are skipped
+    float u;              // inlet velocity when u_tab is null
+    int32_t active;       // 0: the case's work-groups return at once
+    int32_t t_inc;        // coarse steps per engine step (2^(depth offset))
+    int32_t batch;        // ramp batch length in coarse steps (u_tab granularity)
+    const float* u_tab;   // inlet velocity per ramp batch, or null
     float c_wale, nu_sgs_bg, inlet_turb;
     float omega_bulk, omega3, omega4, lambda;
     int32_t adaptive, limiter, comp, wall_model, sponge_blend;
-    int32_t seed;
-    int32_t is_l1;
-};
-
-struct ThermalArgs {
-    const float* g_in;
-    float* g_out;
-    const float* vel;  // the flow step's vel_out
-    float* T_out;
-    const uint8_t* obst;
-    const float* sponge;
-    const int32_t* nbr;
-    const int32_t* borg;
-    const int32_t* order;
-    int32_t nb;
-    int32_t nxg, nyg, nzg;
-    float tau_g;
-    float u_inlet, T_inlet, T_wall;
-    int32_t adiabatic, sponge_blend;
-    int32_t is_l1;
+    float T_inlet, T_wall;
+    int32_t adiabatic, temporal;
 };
 
 // Bouzidi link list: corrected slot dst = a*f[s1] + b*f[s2], all indices
@@ -74,27 +58,45 @@ struct BouzidiLinks {
     float* tmp;
 };
 
-// Ghost blocks: in-domain positions next to a level's blocks that the level
-// does not hold.  Every link pulled from such a position takes the
-// coarse-to-fine interpolation, a function of the source cell and direction
-// only, so it is evaluated once per (ghost cell, direction) into the ghost
-// block's slots before the level's stream-collide.
-struct GhostArgs {
-    void* f;                   // populations being pulled (ghost blocks at index >= nb)
-    int32_t storage;
-    const int32_t* entries;    // [n][2]: (ghost block << 9 | cell, 27-bit direction mask)
-    int64_t n;
-    const int32_t* borg;       // block origins, ghost blocks included
-    const void* a_new;
-    const float* s_new;
-    const float* u_new;
-    const void* a_old;
-    const float* s_old;
-    const float* u_old;
+struct LevelArgs {
+    void* f[2];  // ping-pong pairs, indexed by substep parity (input = t_sub & 1)
+    float* rho[2];
+    float* vel[2];
+    float* g[2];  // thermal populations (FP32) or null
+    float* T[2];
+    const uint8_t* obst;
+    const float* sponge;
+    const float* wall_dist;  // null when the level has no wall distances
+    const int32_t* nbr;
+    const int32_t* borg;
+    int32_t nb;
+    int32_t nxg, nyg, nzg;
+    int32_t lshift;  // level_id - 1
+    float tau, tau_g;
+    // parent level (level_id > 1): its pairs, indexed by the parent's parity
+    const void* pf[2];
+    const float* prho[2];
+    const float* pvel[2];
+    const float* pg[2];
+    const float* pT[2];
     const int32_t* pgrid;
     int32_t pdx, pdy, pdz;
-    float tau_fine, tau_coarse, tw;
-    int32_t temporal, comp, thermal;
+    float ptau, ptau_g;
+    // ghost entries: [n][2] = (ghost block << 9 | cell, 27-bit direction mask)
+    const int32_t* gent;
+    int64_t ngent;
+    BouzidiLinks links;
+};
+
+// Work list entry.  The neighbour row travels with the entry so that the
+// kernel's first dependent loads (LevelArgs, CaseState, neighbour row) are
+// issued together.
+struct Work {
+    int32_t li;              // LevelArgs index
+    int32_t item;            // block (stream-collide) or first entry of a 256-entry chunk
+    int32_t ci;              // CaseState index
+    int32_t pad;
+    const int32_t* nbr_row;  // the block's neighbour row (stream-collide), else null
 };
 
 struct ForceArgs {
@@ -131,10 +133,16 @@ struct ForceArgs {
 
 #define LBK_DECLARE(NS)                                                                                  \
     namespace NS {                                                                                       \
-    hipError_t collide(const lbk::CollideArgs& a, hipStream_t s);                                        \
-    hipError_t thermal(const lbk::ThermalArgs& a, hipStream_t s);                                        \
-    hipError_t bouzidi(const lbk::BouzidiLinks& l, void* f, int storage, hipStream_t s);                 \
-    hipError_t ghost_fill(const lbk::GhostArgs& g, hipStream_t s);                                       \
+    hipError_t collide(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work, int32_t n_tiles, int64_t j, int storage, \
+                       int wide, int sched, hipStream_t s);                                              \
+    hipError_t thermal(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work, int32_t n_tiles, int64_t j,      \
+                       int wide, int sched, hipStream_t s);                                              \
+    hipError_t ghost_fill(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work, int32_t n_chunks, int64_t j,  \
+                          int storage, int thermal, hipStream_t s);                                      \
+    hipError_t bouzidi(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work, int32_t n_chunks, int64_t j,     \
+                       int storage, int thermal, hipStream_t s);                                         \
+    hipError_t set_state(lbk::CaseState* st, const lbk::CaseState& v, hipStream_t s);                   \
+    hipError_t advance(lbk::CaseState* st, int32_t n, hipStream_t s);                             \
     hipError_t init_level(void* f0, void* f1, float* rho0, float* rho1, float* v0, float* v1, int32_t nb,  \
                           int storage, hipStream_t s);                                                   \
     hipError_t init_thermal(float* g0, float* g1, float* T0, float* T1, int32_t nb, float Tinit,         \

@@ -47,6 +47,192 @@ __host__ __device__ constexpr float WK(int k) {
 
 __device__ __forceinline__ size_t fidx(int b, int k, int c) { return ((size_t)b * Q + k) * NC + c; }
 
+// Per-launch views of one level at one substep, resolved from LevelArgs and
+// CaseState at kernel entry (the buffers of the substep's parity, the seed,
+// the inlet velocity).  All fields are uniform over the work-group.
+struct CollideArgs {
+    const void* f_in;
+    void* f_out;
+    const float* vel_in;
+    float* vel_out;
+    float* rho_out;
+    const uint8_t* obst;
+    const float* sponge;
+    const float* wall_dist;
+    const int32_t* nbr;
+    const int32_t* borg;
+    int32_t nb;
+    int32_t nxg, nyg, nzg;
+    float tau;
+    float u_inlet;
+    float c_wale, nu_sgs_bg, inlet_turb;
+    float omega_bulk, omega3, omega4, lambda;
+    int32_t adaptive, limiter, comp, wall_model, sponge_blend;
+    int32_t seed;
+    int32_t is_l1;
+};
+
+struct ThermalArgs {
+    const float* g_in;
+    float* g_out;
+    const float* vel;  // the flow step's vel_out
+    float* T_out;
+    const uint8_t* obst;
+    const float* sponge;
+    const int32_t* nbr;
+    const int32_t* borg;
+    int32_t nb;
+    int32_t nxg, nyg, nzg;
+    float tau_g;
+    float u_inlet, T_inlet, T_wall;
+    int32_t adiabatic, sponge_blend;
+    int32_t is_l1;
+};
+
+struct GhostArgs {
+    void* f;  // populations being pulled (ghost blocks at index >= nb)
+    const int32_t* entries;
+    int64_t n;
+    const int32_t* borg;
+    const void* a_new;
+    const float* s_new;
+    const float* u_new;
+    const void* a_old;
+    const float* s_old;
+    const float* u_old;
+    const int32_t* pgrid;
+    int32_t pdx, pdy, pdz;
+    float tau_fine, tau_coarse, tw;
+    int32_t temporal, comp, thermal;
+};
+
+// The case record, read through the global address space (uniform loads).
+__device__ __forceinline__ CaseState case_state(const CaseState* st) {
+    static_assert(sizeof(CaseState) % 4 == 0, "CaseState is read as 32-bit words");
+    typedef const __attribute__((address_space(1))) uint32_t GW;
+    const GW* src = (const GW*)st;
+    CaseState out;
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(CaseState) / 4; i++) dst[i] = src[i];
+    return out;
+}
+
+__device__ __forceinline__ int64_t substep(const LevelArgs& L, const CaseState& S, int64_t j) {
+    return (S.t0 << L.lshift) + j;
+}
+
+// Coarse step that substep j of the level belongs to.  An engine step can
+// cover several coarse steps of a case; those past the case's last step are
+// skipped (the case then idles until it is removed).
+__device__ __forceinline__ int64_t coarse_step(const LevelArgs& L, const CaseState& S, int64_t j) {
+    return S.t0 + (j >> L.lshift);
+}
+__device__ __forceinline__ bool live(const LevelArgs& L, const CaseState& S, int64_t j) {
+    return S.active != 0 && coarse_step(L, S, j) <= S.n_total;
+}
+// Inlet velocity of the coarse step: the ramp table (engine, one value per
+// batch of S.batch coarse steps, main.jl:214-218) or the value of the call.
+__device__ __forceinline__ float inlet_u(const LevelArgs& L, const CaseState& S, int64_t j) {
+    if (S.u_tab == nullptr) return S.u;
+    const int64_t b = (coarse_step(L, S, j) - 1) / S.batch;
+    return ((const __attribute__((address_space(1))) float*)S.u_tab)[b];
+}
+
+__device__ __forceinline__ CollideArgs collide_view(const LevelArgs& L, const CaseState& S, int64_t j) {
+    const int64_t ts = substep(L, S, j);
+    const int par = (int)(ts & 1);
+    CollideArgs a;
+    a.f_in = L.f[par];
+    a.f_out = L.f[1 - par];
+    a.vel_in = L.vel[par];
+    a.vel_out = L.vel[1 - par];
+    a.rho_out = L.rho[1 - par];
+    a.obst = L.obst;
+    a.sponge = L.sponge;
+    a.wall_dist = L.wall_dist;
+    a.nbr = L.nbr;
+    a.borg = L.borg;
+    a.nb = L.nb;
+    a.nxg = L.nxg;
+    a.nyg = L.nyg;
+    a.nzg = L.nzg;
+    a.tau = L.tau;
+    a.u_inlet = inlet_u(L, S, j);
+    a.c_wale = S.c_wale;
+    a.nu_sgs_bg = S.nu_sgs_bg;
+    a.inlet_turb = S.inlet_turb;
+    a.omega_bulk = S.omega_bulk;
+    a.omega3 = S.omega3;
+    a.omega4 = S.omega4;
+    a.lambda = S.lambda;
+    a.adaptive = S.adaptive;
+    a.limiter = S.limiter;
+    a.comp = S.comp;
+    a.wall_model = (S.wall_model == 1 && L.wall_dist != nullptr) ? 1 : 0;
+    a.sponge_blend = S.sponge_blend;
+    a.seed = (int32_t)(ts % 1000000);
+    a.is_l1 = L.lshift == 0 ? 1 : 0;
+    return a;
+}
+
+__device__ __forceinline__ ThermalArgs thermal_view(const LevelArgs& L, const CaseState& S, int64_t j) {
+    const int par = (int)(substep(L, S, j) & 1);
+    ThermalArgs t;
+    t.g_in = L.g[par];
+    t.g_out = L.g[1 - par];
+    t.vel = L.vel[1 - par];
+    t.T_out = L.T[1 - par];
+    t.obst = L.obst;
+    t.sponge = L.sponge;
+    t.nbr = L.nbr;
+    t.borg = L.borg;
+    t.nb = L.nb;
+    t.nxg = L.nxg;
+    t.nyg = L.nyg;
+    t.nzg = L.nzg;
+    t.tau_g = L.tau_g;
+    t.u_inlet = inlet_u(L, S, j);
+    t.T_inlet = S.T_inlet;
+    t.T_wall = S.T_wall;
+    t.adiabatic = S.adiabatic;
+    t.sponge_blend = S.sponge_blend;
+    t.is_l1 = L.lshift == 0 ? 1 : 0;
+    return t;
+}
+
+// Coarse-to-fine source of substep j (level > 1): the parent's last step is
+// substep j >> 1 of the parent level; its output pair half is "new", its
+// input half (kept untouched until the parent's next step) is "old".  The
+// temporal weight is 0 for the first child substep and 0.5 for the second
+// (solver_control.jl:66-96).
+__device__ __forceinline__ GhostArgs ghost_view(const LevelArgs& L, const CaseState& S, int64_t j, int thermal) {
+    const int par = (int)(substep(L, S, j) & 1);
+    const int ppar = (int)(((S.t0 << (L.lshift - 1)) + (j >> 1)) & 1);
+    GhostArgs g;
+    g.f = thermal ? (void*)L.g[par] : L.f[par];
+    g.entries = L.gent;
+    g.n = L.ngent;
+    g.borg = L.borg;
+    g.a_new = thermal ? (const void*)L.pg[1 - ppar] : L.pf[1 - ppar];
+    g.s_new = thermal ? L.pT[1 - ppar] : L.prho[1 - ppar];
+    g.u_new = L.pvel[1 - ppar];
+    g.a_old = thermal ? (const void*)L.pg[ppar] : L.pf[ppar];
+    g.s_old = thermal ? L.pT[ppar] : L.prho[ppar];
+    g.u_old = L.pvel[ppar];
+    g.pgrid = L.pgrid;
+    g.pdx = L.pdx;
+    g.pdy = L.pdy;
+    g.pdz = L.pdz;
+    g.tau_fine = thermal ? L.tau_g : L.tau;
+    g.tau_coarse = thermal ? L.ptau_g : L.ptau;
+    g.tw = (j & 1) ? 0.5f : 0.0f;
+    g.temporal = S.temporal;
+    g.comp = S.comp;
+    g.thermal = thermal;
+    return g;
+}
+
 // Global-address-space views of the argument pointers, so that loads and
 // stores compile to global_* (not flat_*) instructions.
 typedef __attribute__((address_space(1))) float GF;
@@ -55,6 +241,7 @@ __device__ __forceinline__ const GF* gp(const float* p) { return (const GF*)p; }
 __device__ __forceinline__ GF* gp(float* p) { return (GF*)p; }
 __device__ __forceinline__ const GU8* gp(const uint8_t* p) { return (const GU8*)p; }
 typedef __attribute__((address_space(1))) uint16_t GU16;
+typedef __attribute__((address_space(1))) char GC;
 
 // Population storage.  ST = 0: FP32.  ST = 1 / 2: the offset f - w_k in
 // binary16 / bfloat16, round-to-nearest-even; all arithmetic stays FP32 and a
@@ -66,12 +253,21 @@ struct Codec<0> {
     static __device__ __forceinline__ float load(const void* base, size_t i, float) {
         return ((const GF*)base)[i];
     }
+    template <class OffT>
+    static __device__ __forceinline__ float loadb(const void* base, OffT off, float) {
+        return *(const GF*)((const GC*)base + off);
+    }
     static __device__ __forceinline__ void store(void* base, size_t i, float f, float) { ((GF*)base)[i] = f; }
 };
 template <>
 struct Codec<1> {
     static __device__ __forceinline__ float load(const void* base, size_t i, float w) {
         const uint16_t u = ((const GU16*)base)[i];
+        return w + __half2float(__ushort_as_half(u));
+    }
+    template <class OffT>
+    static __device__ __forceinline__ float loadb(const void* base, OffT off, float w) {
+        const uint16_t u = *(const GU16*)((const GC*)base + off);
         return w + __half2float(__ushort_as_half(u));
     }
     static __device__ __forceinline__ void store(void* base, size_t i, float f, float w) {
@@ -82,6 +278,11 @@ template <>
 struct Codec<2> {
     static __device__ __forceinline__ float load(const void* base, size_t i, float w) {
         const uint32_t u = ((const GU16*)base)[i];
+        return w + __uint_as_float(u << 16);
+    }
+    template <class OffT>
+    static __device__ __forceinline__ float loadb(const void* base, OffT off, float w) {
+        const uint32_t u = *(const GU16*)((const GC*)base + off);
         return w + __uint_as_float(u << 16);
     }
     static __device__ __forceinline__ uint16_t enc(float x) {
@@ -184,29 +385,137 @@ __device__ __forceinline__ void chim_bwd(float& k0, float& k1, float& k2, float 
 // of the level's processing order (blocks sorted by Morton code, two
 // work-groups per block), so a block and most of its neighbours share an L2
 // and are processed close in time.  Returns false for padding work-groups.
-__device__ __forceinline__ bool schedule(const int32_t* __restrict__ order, int nb, int& b, int& half) {
+// sched = 1 maps work-group i to half-tile i (no XCD awareness).
+__device__ __forceinline__ bool schedule(const Work* __restrict__ work, int n_tiles, int sched, Work& wk,
+                                         int& half) {
     const unsigned G = gridDim.x;            // multiple of 8
     const unsigned w = blockIdx.x;
-    const unsigned logical = (w & 7u) * (G >> 3) + (w >> 3);
-    if (logical >= 2u * (unsigned)nb) return false;
-    b = order[logical >> 1];
+    const unsigned logical = sched == 1 ? w : (w & 7u) * (G >> 3) + (w >> 3);
+    if (logical >= 2u * (unsigned)n_tiles) return false;
+    wk = work[logical >> 1];
     half = (int)(logical & 1u);
     return true;
+}
+
+// Pull tables.  A wave is one z-plane of its block, so for a pulled direction
+// the z-side of the source block is the same for the whole wave and only the
+// x/y side varies by lane.  For each of the work-group's 4 waves and each
+// entry e = 9(cz+1) + (ox+1) + 3(oy+1) (ox/oy: x/y side of the source block
+// seen by the lane), ptab holds a byte offset such that
+//     ptab[e] + esz * (c + 512 k - (cx + 8 cy + 64 cz))
+// is the source slot (k, wrapped source cell) in that block; the cell index c
+// and the direction constant complete it with one add per load.  A missing
+// block (no neighbour, outside the domain) maps to an in-bounds slot of the
+// cell's own block and sets bit e of the wave's missing mask; the boundary
+// cascade replaces the value.  Offsets are computed modulo 2^OffT bits; every final offset is
+// in [0, buffer size), which the host guarantees fits OffT.
+template <class OffT>
+__device__ __forceinline__ void build_pull_table(const int32_t* __restrict__ row, int b, int half, OffT esz,
+                                                 OffT* ptab, uint64_t* smask) {
+    const int t = threadIdx.x;
+    bool missing = false;
+    if (t < 108) {
+        const int w = t / 27, e = t - 27 * w;
+        const int cz = e / 9 - 1, j = e - 9 * (cz + 1);
+        const int ox = j % 3 - 1, oy = j / 3 - 1;
+        const int zw = half * 4 + w;
+        const int oz = (cz == 1 && zw == 0) ? -1 : ((cz == -1 && zw == 7) ? 1 : 0);
+        const int nbb = row[j + 9 * (oz + 1)];
+        const OffT bs = (OffT)(Q * NC);
+        OffT v;
+        if (nbb >= 0) {
+            v = ((OffT)nbb * bs - (OffT)(8 * ox + 64 * oy + 512 * oz)) * esz;
+        } else {
+            v = ((OffT)b * bs + (OffT)128) * esz;
+            missing = true;
+        }
+        ptab[t] = v;
+    }
+    // entries 0..107 live in waves 0 and 1: their ballots form a 128-bit mask
+    const uint64_t m = __ballot(missing);
+    if (t == 0) smask[0] = m;
+    if (t == 64) smask[1] = m;
+}
+
+// The 27 missing-block bits of wave w (entries 27w .. 27w+26 of the mask).
+__device__ __forceinline__ uint32_t wave_miss(const uint64_t* smask, int w) {
+    const uint64_t lo = smask[0], hi = smask[1];
+    const int o = 27 * w;
+    const uint64_t v = o == 0 ? lo : (o < 64 ? (lo >> o) | (hi << (64 - o)) : hi >> (o - 64));
+    return (uint32_t)(v & 0x7ffffffu);
+}
+
+// Velocity stencil table (WALE neighbours, physics_utils.jl:112-123): per
+// wave and face n (+x, -x, +y, -y, +z, -z), the byte offset for lanes whose
+// neighbour lies in the next block (x = 7 for +x, ...; for +-z the whole wave
+// or none), such that vtab + 4 (c + dx + 8 dy + 64 dz) addresses component 0;
+// entry 6 is the own block.  Neighbours outside the level (no block, or a
+// ghost block, which holds no velocity) use the cell's own velocity.
+template <class OffT>
+__device__ __forceinline__ void build_vel_table(const int32_t* __restrict__ row, int nb, int b, int half,
+                                                OffT* vtab) {
+    const int t = threadIdx.x - 128;
+    if (t >= 0 && t < 32) {
+        const int w = t >> 3, n = t & 7;
+        const OffT own = (OffT)b * (OffT)(3 * NC * 4);
+        OffT v = own;
+        if (n < 6) {
+            const int dx = n == 0 ? 1 : (n == 1 ? -1 : 0);
+            const int dy = n == 2 ? 1 : (n == 3 ? -1 : 0);
+            const int dz = n == 4 ? 1 : (n == 5 ? -1 : 0);
+            const int zw = half * 4 + w;
+            const bool zedge = (dz == 1 && zw == 7) || (dz == -1 && zw == 0);
+            if (dz == 0 || zedge) {
+                const int nbb = row[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+                if (nbb >= 0 && nbb < nb)
+                    v = (OffT)nbb * (OffT)(3 * NC * 4) - (OffT)(4 * (8 * dx + 64 * dy + 512 * dz));
+                else
+                    v = own - (OffT)(4 * (dx + 8 * dy + 64 * dz));
+            }
+        }
+        vtab[t] = v;
+    }
+}
+
+template <class OffT>
+__device__ __forceinline__ const GF* gaddr(const void* base, OffT off) {
+    return (const GF*)((const GC*)base + off);
 }
 
 // stream_collide_cumulant! (src/collision/kernel_cumulant.jl:29-627), with the
 // A/B buffers chosen by the caller and no f_post_collision copy (Bouzidi
 // reads f_out before it corrects it, see bouzidi_gather below).
-template <int ST>
-__global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
-    __shared__ int snbr[27];
+// Occupancy: the collision needs ~160 VGPRs, which allows 3 waves per SIMD.
+// Capping the kernel at 96 VGPRs (5 waves per SIMD; 5 work-groups of 28 KB
+// LDS per CU) spills ~50 values per lane to scratch, which stays in cache,
+// and hides more of the pull's memory latency: measured 20 % faster with
+// FP16 storage and equal with FP32 on the 52.5M-cell cube case.
+template <int ST, class OffT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5)))
+collide_kernel(const LevelArgs* __restrict__ las,
+                                                      const CaseState* __restrict__ states,
+                                                      const Work* __restrict__ work, int32_t n_tiles, int64_t j,
+                                                      int32_t sched) {
+    __shared__ OffT ptab[108];
+    __shared__ OffT vtab[32];
+    __shared__ uint64_t smask[2];
     __shared__ float sfix[27 * 256];
-    int b, half;
-    if (!schedule(a.order, a.nb, b, half)) return;
+    Work wk;
+    int half;
+    if (!schedule(work, n_tiles, sched, wk, half)) return;
+    const LevelArgs& L = las[wk.li];
+    const int b = wk.item;
+    constexpr int ESZ = ST == 0 ? 4 : 2;
     const int c = (half << 8) | threadIdx.x;
-    if (threadIdx.x < 27) snbr[threadIdx.x] = a.nbr[(size_t)b * 27 + threadIdx.x];
+    // the neighbour-row loads go out together with the case record's
+    build_pull_table<OffT>(wk.nbr_row, b, half, (OffT)ESZ, ptab, smask);
+    build_vel_table<OffT>(wk.nbr_row, L.nb, b, half, vtab);
+    const CaseState S = case_state(states + wk.ci);
     __syncthreads();
+    if (!live(L, S, j)) return;
+    const CollideArgs a = collide_view(L, S, j);
     const int x = c & 7, y = (c >> 3) & 7, z = c >> 6;
+    const int wv = threadIdx.x >> 6;
     const int gx = a.borg[b * 3 + 0] + x + 1;
     const int gy = a.borg[b * 3 + 1] + y + 1;
     const int gz = a.borg[b * 3 + 2] + z + 1;
@@ -214,46 +523,47 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
     const bool is_obs = gp(a.obst)[i4] != 0;
 
     // STEP 1: pull (kernel_cumulant.jl:86-168).  All 27 loads are issued
-    // before any is used: a missing neighbour block only swaps in the cell's
-    // own slot as a harmless address, and the boundary cascade then replaces
-    // the value (miss mask), keeping the loads free of branches.
+    // before any is used, each one table read and one add.
+    const int oxp = x == 0 ? -1 : 0, oxm = x == 7 ? 1 : 0;
+    const int oyp = y == 0 ? -1 : 0, oym = y == 7 ? 1 : 0;
+    const OffT cb = (OffT)c * (OffT)ESZ;
     float A[27];
-    uint32_t miss = 0u;
 #pragma unroll
     for (int k = 0; k < 27; k++) {
         const int cx = CX(k), cy = CY(k), cz = CZ(k);
-        const int sx = x - cx, sy = y - cy, sz = z - cz;
-        const int ox = cx == 0 ? 0 : (sx < 0 ? -1 : (sx > 7 ? 1 : 0));
-        const int oy = cy == 0 ? 0 : (sy < 0 ? -1 : (sy > 7 ? 1 : 0));
-        const int oz = cz == 0 ? 0 : (sz < 0 ? -1 : (sz > 7 ? 1 : 0));
-        const int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
-        const int blk = nbb >= 0 ? nbb : b;
-        const int cc = nbb >= 0 ? ((sx & 7) + 8 * (sy & 7) + 64 * (sz & 7)) : c;
-        A[k] = Codec<ST>::load(a.f_in, fidx(blk, k, cc), WK(k));
-        miss |= (nbb < 0 ? 1u : 0u) << k;
+        const int ox = cx == 1 ? oxp : (cx == -1 ? oxm : 0);
+        const int oy = cy == 1 ? oyp : (cy == -1 ? oym : 0);
+        const OffT off = ptab[wv * 27 + 9 * (cz + 1) + (ox + 1) + 3 * (oy + 1)] + cb +
+                         (OffT)((512 * k - (cx + 8 * cy + 64 * cz)) * ESZ);
+        A[k] = Codec<ST>::loadb(a.f_in, off, WK(k));
     }
-    // WALE stencil of the previous velocity (physics_utils.jl:112-123), loaded
-    // together with the populations.
+    // WALE stencil of the previous velocity, loaded together with the populations.
     float VN[6][3];
     {
-        const GF* __restrict__ vin = gp(a.vel_in);
+        const OffT own = vtab[wv * 8 + 6];
 #pragma unroll
         for (int n = 0; n < 6; n++) {
             const int dx = n == 0 ? 1 : (n == 1 ? -1 : 0);
             const int dy = n == 2 ? 1 : (n == 3 ? -1 : 0);
             const int dz = n == 4 ? 1 : (n == 5 ? -1 : 0);
-            const int nx = x + dx, ny = y + dy, nz = z + dz;
-            const int ox = dx == 0 ? 0 : (nx < 0 ? -1 : (nx > 7 ? 1 : 0));
-            const int oy = dy == 0 ? 0 : (ny < 0 ? -1 : (ny > 7 ? 1 : 0));
-            const int oz = dz == 0 ? 0 : (nz < 0 ? -1 : (nz > 7 ? 1 : 0));
-            const int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
-            const bool real = nbb >= 0 && nbb < a.nb;   // ghost blocks hold no velocity
-            const int blk = real ? nbb : b;
-            const int cc = real ? ((nx & 7) + 8 * (ny & 7) + 64 * (nz & 7)) : c;
-            const size_t iv = (size_t)blk * 3 * NC + cc;
-            VN[n][0] = vin[iv];
-            VN[n][1] = vin[iv + NC];
-            VN[n][2] = vin[iv + 2 * NC];
+            const bool edge = n == 0 ? x == 7 : n == 1 ? x == 0 : n == 2 ? y == 7 : n == 3 ? y == 0 : true;
+            const OffT off = (edge ? vtab[wv * 8 + n] : own) + (OffT)(4 * (c + dx + 8 * dy + 64 * dz));
+            const GF* p = gaddr(a.vel_in, off);
+            VN[n][0] = p[0];
+            VN[n][1] = p[NC];
+            VN[n][2] = p[2 * NC];
+        }
+    }
+    // lanes whose pull crosses into a missing block (domain boundary)
+    uint32_t miss = 0u;
+    const uint32_t m = wave_miss(smask, wv);
+    if (m != 0u) {
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+            const int cx = CX(k), cy = CY(k), cz = CZ(k);
+            const int ox = cx == 1 ? oxp : (cx == -1 ? oxm : 0);
+            const int oy = cy == 1 ? oyp : (cy == -1 ? oym : 0);
+            miss |= ((m >> (9 * (cz + 1) + (ox + 1) + 3 * (oy + 1))) & 1u) << k;
         }
     }
     const float sp = gp(a.sponge)[i4];
@@ -568,11 +878,19 @@ __global__ void __launch_bounds__(256) collide_kernel(const CollideArgs a) {
 
 inline unsigned padded_groups(int nb) { return ((unsigned)nb * 2u + 7u) & ~7u; }
 
-hipError_t collide(const CollideArgs& a, hipStream_t s) {
-    if (a.nb <= 0) return hipSuccess;
-    if (a.storage == 1) hipLaunchKernelGGL(collide_kernel<1>, dim3(padded_groups(a.nb)), dim3(256), 0, s, a);
-    else if (a.storage == 2) hipLaunchKernelGGL(collide_kernel<2>, dim3(padded_groups(a.nb)), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(collide_kernel<0>, dim3(padded_groups(a.nb)), dim3(256), 0, s, a);
+hipError_t collide(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_tiles, int64_t j, int storage, int wide,
+                   int sched, hipStream_t s) {
+    if (n_tiles <= 0) return hipSuccess;
+    const dim3 g(padded_groups(n_tiles)), t(256);
+    if (wide) {
+        if (storage == 1) hipLaunchKernelGGL((collide_kernel<1, uint64_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+        else if (storage == 2) hipLaunchKernelGGL((collide_kernel<2, uint64_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+        else hipLaunchKernelGGL((collide_kernel<0, uint64_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+    } else {
+        if (storage == 1) hipLaunchKernelGGL((collide_kernel<1, uint32_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+        else if (storage == 2) hipLaunchKernelGGL((collide_kernel<2, uint32_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+        else hipLaunchKernelGGL((collide_kernel<0, uint32_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+    }
     return hipGetLastError();
 }
 
@@ -588,15 +906,27 @@ __device__ __forceinline__ float thermal_bc(const ThermalArgs& a, int k, int b, 
     return w * a.T_inlet;
 }
 
-__global__ void __launch_bounds__(256) thermal_kernel(const ThermalArgs a) {
-    __shared__ int snbr[27];
+template <class OffT>
+__global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restrict__ las,
+                                                      const CaseState* __restrict__ states,
+                                                      const Work* __restrict__ work, int32_t n_tiles, int64_t j,
+                                                      int32_t sched) {
+    __shared__ OffT ptab[108];
+    __shared__ uint64_t smask[2];
     __shared__ float sfix[27 * 256];
-    int b, half;
-    if (!schedule(a.order, a.nb, b, half)) return;
+    Work wk;
+    int half;
+    if (!schedule(work, n_tiles, sched, wk, half)) return;
+    const LevelArgs& L = las[wk.li];
+    const int b = wk.item;
     const int c = (half << 8) | threadIdx.x;
-    if (threadIdx.x < 27) snbr[threadIdx.x] = a.nbr[(size_t)b * 27 + threadIdx.x];
+    build_pull_table<OffT>(wk.nbr_row, b, half, (OffT)4, ptab, smask);
+    const CaseState S = case_state(states + wk.ci);
     __syncthreads();
+    if (!live(L, S, j)) return;
+    const ThermalArgs a = thermal_view(L, S, j);
     const int x = c & 7, y = (c >> 3) & 7, z = c >> 6;
+    const int wv = threadIdx.x >> 6;
     const int gx = a.borg[b * 3 + 0] + x + 1;
     const int gy = a.borg[b * 3 + 1] + y + 1;
     const int gz = a.borg[b * 3 + 2] + z + 1;
@@ -604,21 +934,29 @@ __global__ void __launch_bounds__(256) thermal_kernel(const ThermalArgs a) {
     const size_t iv = (size_t)b * 3 * NC + c;
     const bool is_obs = gp(a.obst)[i4] != 0;
     float ux = gp(a.vel)[iv], uy = gp(a.vel)[iv + NC], uz = gp(a.vel)[iv + 2 * NC];
-    const GF* __restrict__ gin = gp(a.g_in);
+    const int oxp = x == 0 ? -1 : 0, oxm = x == 7 ? 1 : 0;
+    const int oyp = y == 0 ? -1 : 0, oym = y == 7 ? 1 : 0;
+    const OffT cb = (OffT)c * (OffT)4;
     float G[27];
-    uint32_t miss = 0u;
 #pragma unroll
     for (int k = 0; k < 27; k++) {
         const int cx = CX(k), cy = CY(k), cz = CZ(k);
-        const int sx = x - cx, sy = y - cy, sz = z - cz;
-        const int ox = cx == 0 ? 0 : (sx < 0 ? -1 : (sx > 7 ? 1 : 0));
-        const int oy = cy == 0 ? 0 : (sy < 0 ? -1 : (sy > 7 ? 1 : 0));
-        const int oz = cz == 0 ? 0 : (sz < 0 ? -1 : (sz > 7 ? 1 : 0));
-        const int nbb = snbr[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
-        const int blk = nbb >= 0 ? nbb : b;
-        const int cc = nbb >= 0 ? ((sx & 7) + 8 * (sy & 7) + 64 * (sz & 7)) : c;
-        G[k] = gin[fidx(blk, k, cc)];
-        miss |= (nbb < 0 ? 1u : 0u) << k;
+        const int ox = cx == 1 ? oxp : (cx == -1 ? oxm : 0);
+        const int oy = cy == 1 ? oyp : (cy == -1 ? oym : 0);
+        const OffT off = ptab[wv * 27 + 9 * (cz + 1) + (ox + 1) + 3 * (oy + 1)] + cb +
+                         (OffT)((512 * k - (cx + 8 * cy + 64 * cz)) * 4);
+        G[k] = *gaddr(a.g_in, off);
+    }
+    uint32_t miss = 0u;
+    const uint32_t m = wave_miss(smask, wv);
+    if (m != 0u) {
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+            const int cx = CX(k), cy = CY(k), cz = CZ(k);
+            const int ox = cx == 1 ? oxp : (cx == -1 ? oxm : 0);
+            const int oy = cy == 1 ? oyp : (cy == -1 ? oym : 0);
+            miss |= ((m >> (9 * (cz + 1) + (ox + 1) + 3 * (oy + 1))) & 1u) << k;
+        }
     }
     if (miss != 0u) {
 #pragma unroll 1
@@ -669,9 +1007,12 @@ __global__ void __launch_bounds__(256) thermal_kernel(const ThermalArgs a) {
     }
 }
 
-hipError_t thermal(const ThermalArgs& a, hipStream_t s) {
-    if (a.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(thermal_kernel, dim3(padded_groups(a.nb)), dim3(256), 0, s, a);
+hipError_t thermal(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_tiles, int64_t j, int wide, int sched,
+                   hipStream_t s) {
+    if (n_tiles <= 0) return hipSuccess;
+    const dim3 g(padded_groups(n_tiles)), t(256);
+    if (wide) hipLaunchKernelGGL(thermal_kernel<uint64_t>, g, t, 0, s, las, states, work, n_tiles, j, sched);
+    else hipLaunchKernelGGL(thermal_kernel<uint32_t>, g, t, 0, s, las, states, work, n_tiles, j, sched);
     return hipGetLastError();
 }
 
@@ -683,8 +1024,15 @@ hipError_t thermal(const ThermalArgs& a, hipStream_t s) {
 // direction then blends its own population component.  Per value, the
 // operations are those of the reference function, in its order.
 template <int ST>
-__global__ void __launch_bounds__(256) ghost_kernel(const GhostArgs a) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) ghost_kernel(const LevelArgs* __restrict__ las,
+                                                    const CaseState* __restrict__ states,
+                                                    const Work* __restrict__ work, int64_t j, int thermal) {
+    const Work wk = work[blockIdx.x];
+    const LevelArgs& L = las[wk.li];
+    const CaseState S = case_state(states + wk.ci);
+    if (!live(L, S, j)) return;
+    const GhostArgs a = ghost_view(L, S, j, thermal);
+    const int64_t i = (int64_t)wk.item + threadIdx.x;
     if (i >= a.n) return;
     const int e0 = a.entries[2 * i];
     const uint32_t mask = (uint32_t)a.entries[2 * i + 1];
@@ -795,12 +1143,16 @@ __global__ void __launch_bounds__(256) ghost_kernel(const GhostArgs a) {
     }
 }
 
-hipError_t ghost_fill(const GhostArgs& g, hipStream_t s) {
-    if (g.n <= 0) return hipSuccess;
-    const dim3 grid((unsigned)((g.n + 255) / 256));
-    if (g.storage == 1) hipLaunchKernelGGL(ghost_kernel<1>, grid, dim3(256), 0, s, g);
-    else if (g.storage == 2) hipLaunchKernelGGL(ghost_kernel<2>, grid, dim3(256), 0, s, g);
-    else hipLaunchKernelGGL(ghost_kernel<0>, grid, dim3(256), 0, s, g);
+// One work-group per work entry (LevelArgs index, first of 256 ghost entries).
+// Thermal populations are FP32 whatever the flow storage.
+hipError_t ghost_fill(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_chunks, int64_t j, int storage, int thermal,
+                      hipStream_t s) {
+    if (n_chunks <= 0) return hipSuccess;
+    const dim3 grid((unsigned)n_chunks), t(256);
+    const int st = thermal ? 0 : storage;
+    if (st == 1) hipLaunchKernelGGL(ghost_kernel<1>, grid, t, 0, s, las, states, work, j, thermal);
+    else if (st == 2) hipLaunchKernelGGL(ghost_kernel<2>, grid, t, 0, s, las, states, work, j, thermal);
+    else hipLaunchKernelGGL(ghost_kernel<0>, grid, t, 0, s, las, states, work, j, thermal);
     return hipGetLastError();
 }
 
@@ -813,34 +1165,83 @@ hipError_t ghost_fill(const GhostArgs& g, hipStream_t s) {
 // products and the sum rounded separately in both builds.
 // All three slots of a link have the same |c|, so one weight w_k (from the
 // destination's direction) decodes and encodes them.
+// The corrected buffer is the substep's output half (f_out, or g_out).
+__device__ __forceinline__ void* bouzidi_target(const LevelArgs& L, const CaseState& S, int64_t j, int thermal) {
+    const int out = 1 - (int)(substep(L, S, j) & 1);
+    return thermal ? (void*)L.g[out] : L.f[out];
+}
 template <int ST>
-__global__ void bouzidi_gather(const BouzidiLinks l, const void* __restrict__ f) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) bouzidi_gather(const LevelArgs* __restrict__ las,
+                                                      const CaseState* __restrict__ states,
+                                                      const Work* __restrict__ work, int64_t j, int thermal) {
+    const Work wk = work[blockIdx.x];
+    const LevelArgs& L = las[wk.li];
+    const CaseState S = case_state(states + wk.ci);
+    if (!live(L, S, j)) return;
+    const BouzidiLinks& l = L.links;
+    const int64_t i = (int64_t)wk.item + threadIdx.x;
     if (i >= l.n) return;
+    const void* f = bouzidi_target(L, S, j, thermal);
     const float w = wk_rt((int)((l.dst[i] >> 9) % 27));
     const float f1 = Codec<ST>::load(f, (size_t)l.s1[i], w), f2 = Codec<ST>::load(f, (size_t)l.s2[i], w);
     l.tmp[i] = __fadd_rn(__fmul_rn(l.a[i], f1), __fmul_rn(l.b[i], f2));
 }
 template <int ST>
-__global__ void bouzidi_scatter(const BouzidiLinks l, void* __restrict__ f) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) bouzidi_scatter(const LevelArgs* __restrict__ las,
+                                                       const CaseState* __restrict__ states,
+                                                       const Work* __restrict__ work, int64_t j, int thermal) {
+    const Work wk = work[blockIdx.x];
+    const LevelArgs& L = las[wk.li];
+    const CaseState S = case_state(states + wk.ci);
+    if (!live(L, S, j)) return;
+    const BouzidiLinks& l = L.links;
+    const int64_t i = (int64_t)wk.item + threadIdx.x;
     if (i >= l.n) return;
+    void* f = bouzidi_target(L, S, j, thermal);
     const float w = wk_rt((int)((l.dst[i] >> 9) % 27));
     Codec<ST>::store(f, (size_t)l.dst[i], l.tmp[i], w);
 }
 
 template <int ST>
-void bouzidi_launch(const BouzidiLinks& l, void* f, hipStream_t s) {
-    unsigned g = (unsigned)((l.n + 255) / 256);
-    hipLaunchKernelGGL(bouzidi_gather<ST>, dim3(g), dim3(256), 0, s, l, (const void*)f);
-    hipLaunchKernelGGL(bouzidi_scatter<ST>, dim3(g), dim3(256), 0, s, l, f);
+void bouzidi_launch(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_chunks, int64_t j,
+                    int thermal, hipStream_t s) {
+    const dim3 g((unsigned)n_chunks), t(256);
+    hipLaunchKernelGGL(bouzidi_gather<ST>, g, t, 0, s, las, states, work, j, thermal);
+    hipLaunchKernelGGL(bouzidi_scatter<ST>, g, t, 0, s, las, states, work, j, thermal);
 }
 
-hipError_t bouzidi(const BouzidiLinks& l, void* f, int storage, hipStream_t s) {
-    if (l.n <= 0) return hipSuccess;
-    if (storage == 1) bouzidi_launch<1>(l, f, s);
-    else if (storage == 2) bouzidi_launch<2>(l, f, s);
-    else bouzidi_launch<0>(l, f, s);
+// One work-group per work entry (LevelArgs index, first of 256 links).
+hipError_t bouzidi(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_chunks, int64_t j, int storage, int thermal,
+                   hipStream_t s) {
+    if (n_chunks <= 0) return hipSuccess;
+    const int st = thermal ? 0 : storage;
+    if (st == 1) bouzidi_launch<1>(las, states, work, n_chunks, j, thermal, s);
+    else if (st == 2) bouzidi_launch<2>(las, states, work, n_chunks, j, thermal, s);
+    else bouzidi_launch<0>(las, states, work, n_chunks, j, thermal, s);
+    return hipGetLastError();
+}
+
+// Case state updates.  set_state writes a whole record (single-case stepping,
+// once per coarse step); advance moves every case of an engine by its
+// step increment and looks up the inlet velocity of the new ramp batch.
+__global__ void set_state_kernel(CaseState* st, const CaseState v) {
+    if (threadIdx.x == 0) *st = v;
+}
+hipError_t set_state(CaseState* st, const CaseState& v, hipStream_t s) {
+    hipLaunchKernelGGL(set_state_kernel, dim3(1), dim3(64), 0, s, st, v);
+    return hipGetLastError();
+}
+__global__ void advance_kernel(CaseState* st, int32_t n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    CaseState& c = st[i];
+    if (c.active == 0) return;
+    c.t0 += c.t_inc;
+    if (c.t0 > c.n_total) c.active = 0;
+}
+hipError_t advance(CaseState* st, int32_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(advance_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, st, n);
     return hipGetLastError();
 }
 

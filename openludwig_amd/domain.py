@@ -268,8 +268,10 @@ def should_use_bouzidi(level_id: int, num_levels: int, method: str, bouzidi_leve
 
 
 def setup_multilevel_domain(cfg: CaseConfig, mesh: SolverMesh, params: DomainParameters,
-                            verbose: bool = False) -> list:
-    """setup_multilevel_domain(mesh, params) (domain.jl:1-246)."""
+                            verbose: bool = False, topology_only: bool = False) -> list:
+    """setup_multilevel_domain(mesh, params) (domain.jl:1-246).  With
+    topology_only, only the block sets are built and the block count of each
+    level is returned (used to size serving requests)."""
     lib = N.setup_lib()
     off = np.ascontiguousarray(params.mesh_offset, dtype=np.float64)
     placed_min = np.array(params.mesh_min) + off
@@ -285,6 +287,7 @@ def setup_multilevel_domain(cfg: CaseConfig, mesh: SolverMesh, params: DomainPar
     tris = np.ascontiguousarray(mesh.triangles, dtype=np.float64)
     grids = []
     prev_grid = None
+    prev_coords_list = []
     for lvl in range(1, params.num_levels + 1):
         scale = 2 ** (lvl - 1)
         dx = params.dx_coarse / scale
@@ -295,7 +298,7 @@ def setup_multilevel_domain(cfg: CaseConfig, mesh: SolverMesh, params: DomainPar
             if cfg.refinement_strategy != "geometry_first":
                 raise NotImplementedError("refinement strategy 'topology' is not supported")
             active = get_active_blocks_for_level(mesh, dx, off, dims)
-            prev_coords = grids[-1].coords
+            prev_coords = prev_coords_list[-1]
             if cfg.wake_enabled:
                 prev_bs = BLOCK_SIZE * (params.dx_coarse / 2 ** (lvl - 2))
                 c = prev_coords.astype(np.float64)
@@ -319,6 +322,11 @@ def setup_multilevel_domain(cfg: CaseConfig, mesh: SolverMesh, params: DomainPar
         idx = np.argwhere(active)   # lexicographic (bx, by, bz) order, bx most significant
         coords = np.ascontiguousarray(idx.astype(np.int32) + 1)
         nb = coords.shape[0]
+        prev_coords_list.append(coords)
+        if topology_only:
+            grids.append(nb)
+            prev_grid = active
+            continue
         nbt = build_neighbor_table(coords, dims)
         dense = np.asfortranarray(_dense_ptr(coords, dims))
         dense_c = np.ascontiguousarray(dense.ravel(order="F"))

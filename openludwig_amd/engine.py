@@ -1,0 +1,95 @@
+"""Batched stepping of many cases on one GPU (lb_engine_* of libludwig_hip).
+
+The reference runs cases one after another (run_all_cases, main.jl:425-469),
+each through execute_timestep_batch! (solver_control.jl:199-219).  Here
+several cases -- each a Solver built as usual -- are attached to one engine
+and advanced together: every launch covers the same recursion depth of all
+cases over a concatenated block list, and one engine step is replayed as a
+captured hipGraph.  A case's coarse steps, seeds and per-batch inlet
+velocities are exactly those of Solver.run, so its results do not depend on
+the other cases sharing the engine.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .solver import LudwigError, Solver, ramp_velocity
+
+
+def ramp_table(cfg, n_steps: int) -> np.ndarray:
+    """Inlet velocity of each batch of cfg.gpu_async_depth coarse steps,
+    evaluated at the batch end as main.jl:214-218 does."""
+    b = cfg.gpu_async_depth
+    n_batches = (n_steps + b - 1) // b
+    return np.array([ramp_velocity(cfg.u_lattice, min((i + 1) * b, n_steps), cfg.ramp_steps)
+                     for i in range(n_batches)], dtype=np.float32)
+
+
+class Engine:
+    def __init__(self, storage: int = N.LB_STORAGE_FP32, strict_fp: bool = False, device: int = 0):
+        self.lib = N.hip_lib()
+        self._e = C.c_void_p()
+        rc = self.lib.lb_engine_create(C.byref(self._e), device, storage, int(strict_fp))
+        if rc != 0:
+            raise LudwigError(f"lb_engine_create failed ({rc})")
+        self.storage = storage
+        self.cases: dict = {}
+
+    def close(self) -> None:
+        if self._e:
+            for s in list(self.cases):
+                s.engine = None
+            self.cases.clear()
+            self.lib.lb_engine_destroy(self._e)
+            self._e = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = self.lib.lb_engine_last_error(self._e).decode(errors="replace")
+            raise LudwigError(f"{what} failed ({rc}): {msg}")
+
+    def add(self, solver: Solver, n_steps: int) -> None:
+        """Attach an initialised case for coarse steps 1..n_steps."""
+        u = ramp_table(solver.cfg, n_steps)
+        self._check(self.lib.lb_engine_add(self._e, solver._ctx, int(n_steps), N.ptr(u, C.c_float), len(u),
+                                           int(solver.cfg.gpu_async_depth), C.byref(solver.step_params)),
+                    "lb_engine_add")
+        self.cases[solver] = n_steps
+        solver.engine = self
+
+    def remove(self, solver: Solver) -> None:
+        self._check(self.lib.lb_engine_remove(self._e, solver._ctx), "lb_engine_remove")
+        solver.t_next = self.cases.pop(solver) + 1
+        solver.engine = None
+
+    def step(self, n: int = 1) -> None:
+        self._check(self.lib.lb_engine_step(self._e, int(n)), "lb_engine_step")
+
+    def synchronize(self) -> None:
+        self._check(self.lib.lb_engine_synchronize(self._e), "lb_engine_synchronize")
+
+    def progress(self, solver: Solver) -> int:
+        t = C.c_int64()
+        self._check(self.lib.lb_engine_case_progress(self._e, solver._ctx, C.byref(t)), "lb_engine_case_progress")
+        return t.value
+
+    def done(self, solver: Solver) -> bool:
+        return self.progress(solver) >= self.cases[solver]
+
+    def set_options(self, graph: bool = True, schedule: int = N.LB_SCHED_XCD) -> None:
+        self._check(self.lib.lb_engine_set_options(self._e, int(graph), int(schedule)), "lb_engine_set_options")
+
+    def run_to_completion(self) -> None:
+        """Step until every attached case has done its coarse steps."""
+        while any(not self.done(s) for s in self.cases):
+            self.step(1)
+        self.synchronize()

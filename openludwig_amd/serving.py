@@ -1,0 +1,314 @@
+"""Serving of case requests on one GPU: continuous batching against a plain
+sequential loop.
+
+A request names a body (sphere, cube or ellipsoid), a Reynolds number, a
+level count and a grid resolution, and asks for the force coefficients after
+a fixed amount of simulated time.  Two ways to serve a stream of requests:
+
+run_sequential   the reference's way (run_all_cases, main.jl:425-469): one
+                 case at a time -- set up, run through execute_timestep_batch!
+                 in batches of gpu_async_depth, compute forces, free.
+run_batched      continuous batching: set-up (CPU tables, then device upload)
+                 runs on worker threads as requests arrive; a case is
+                 admitted when its exact device footprint (lb_footprint) fits
+                 the HBM budget; admitted cases are attached to one Engine and
+                 advance together in batched launches replayed as a hipGraph;
+                 finished cases are detached, their forces computed and their
+                 memory released while the others keep running.
+
+Latency = finish - arrival.  Every per-case result equals the case run alone
+(the engine's per-case sequence does not depend on the mix).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import threading
+import time
+from collections import deque
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from .config import CaseConfig, f32
+from .domain import compute_domain_from_mesh, setup_multilevel_domain
+from .engine import Engine, ramp_table
+from .geometry import cube, ellipsoid, icosphere, mesh_from_triangles, rotate_mesh, subdivide_mesh
+from .solver import Solver, build_case, footprint
+
+NU_AIR = 1.5e-5
+
+
+@dataclass
+class Request:
+    rid: int
+    body: str                      # "sphere", "cube" or "ellipsoid"
+    axes: tuple = (1.0, 1.0, 1.0)  # ellipsoid full lengths (x = 1)
+    reynolds: float = 1e3
+    levels: int = 3
+    resolution: int = 16
+    finest_steps: int = 256        # simulated time, in finest-level steps
+    arrival: float = 0.0           # seconds after the start of the run
+    cells: int = 0                 # active cells (from the block topology)
+
+    def coarse_steps(self, levels: int) -> int:
+        n = max(1, self.finest_steps >> (levels - 1))
+        return int(math.ceil(n / 8.0) * 8)
+
+
+@dataclass
+class Result:
+    rid: int
+    arrival: float
+    start: float
+    finish: float
+    levels: int
+    cells: int
+    coarse_steps: int
+    lu: int
+    cd: float
+    cl: float
+    device_bytes: int = 0
+
+    @property
+    def latency(self) -> float:
+        return self.finish - self.arrival
+
+
+def request_case(req: Request) -> tuple:
+    """Case configuration and mesh of a request.  Bodies are 1 m long in x;
+    the wall model (WMLES) is on from Re 1e5."""
+    if req.body == "sphere":
+        tris, area = icosphere(4, 0.5), math.pi / 4.0
+    elif req.body == "cube":
+        tris, area = cube(1.0), 1.0
+    elif req.body == "ellipsoid":
+        tris = ellipsoid(4, req.axes)
+        area = math.pi / 4.0 * req.axes[1] * req.axes[2]
+    else:
+        raise ValueError(f"unknown body {req.body}")
+    cfg = CaseConfig(
+        surface_resolution=int(req.resolution), num_levels=int(req.levels), reference_area_full_model=area,
+        reference_chord=1.0, reference_length_for_meshing=1.0, reference_dimension="x",
+        fluid_density=1.225, kinematic_viscosity=NU_AIR, flow_velocity=float(req.reynolds) * NU_AIR,
+        steps=0, ramp_steps=max(8, req.coarse_steps(req.levels) // 2), output_freq=10 ** 9,
+        u_lattice=f32(0.05), c_wale=f32(0.5), tau_min=f32(0.501), inlet_turbulence_intensity=f32(0.0),
+        nu_sgs_background=f32(0.0), limiter="positivity", lambda_param=f32(0.1667),
+        wall_model_enabled=req.reynolds >= 1e5, domain_upstream=3.0, domain_downstream=6.0, domain_lateral=2.5,
+        domain_height=2.5, sponge_thickness=f32(0.10), refinement_margin=2, wake_enabled=True,
+        wake_length=1.0, wake_width_factor=1.0, wake_height_factor=1.0, boundary_method="bouzidi",
+        bouzidi_levels=1, q_min_threshold=f32(0.001), forces_enabled=True, moment_center=(0.25, 0.0, 0.0),
+        diag_freq=10 ** 9, min_coarse_blocks=1,
+    ).validate()
+    mesh = mesh_from_triangles(tris)
+    if req.body == "cube":
+        mesh = subdivide_mesh(mesh, 0.1)
+    return cfg, mesh
+
+
+def topology_cells(req: Request) -> tuple:
+    """(active cells, levels) of a request from its block topology alone."""
+    cfg, mesh = request_case(req)
+    mesh = rotate_mesh(mesh, cfg.alpha_deg, cfg.beta_deg)
+    params = compute_domain_from_mesh(cfg, mesh.min_bounds, mesh.max_bounds)
+    nbs = setup_multilevel_domain(cfg, mesh, params, topology_only=True)
+    return sum(nbs) * 512, len(nbs)
+
+
+def size_request(req: Request, target_cells: float, iters: int = 5) -> Request:
+    """Pick a resolution whose active cell count is close to the target:
+    secant steps on log(cells) against log(resolution), from resolution 16."""
+    levels = req.levels
+    res, best = 16, None
+    pts = []
+    for _ in range(iters):
+        req.resolution, req.levels = res, levels
+        n, lv = topology_cells(req)
+        if best is None or abs(math.log(n / target_cells)) < abs(math.log(best[1] / target_cells)):
+            best = (res, n, lv)
+        pts.append((math.log(res), math.log(n)))
+        if abs(math.log(n / target_cells)) < 0.1:
+            break
+        slope = 2.5
+        if len(pts) >= 2 and abs(pts[-1][0] - pts[-2][0]) > 1e-9:
+            slope = min(max((pts[-1][1] - pts[-2][1]) / (pts[-1][0] - pts[-2][0]), 0.5), 3.0)
+        nxt = int(round(math.exp(pts[-1][0] + (math.log(target_cells) - pts[-1][1]) / slope)))
+        nxt = min(max(nxt, 4), 512)
+        if nxt == res:
+            break
+        res = nxt
+    req.resolution, req.cells, req.levels = best
+    return req
+
+
+def make_requests(n: int, rate: float, cells: tuple = (1e6, 40e6), levels: tuple = (2, 5),
+                  reynolds: tuple = (1e3, 1e7), finest_steps: int = 256, seed_arrivals: int = 0,
+                  seed_bodies: int = 1) -> list:
+    """The serving workload: Poisson arrivals at `rate` per second (seed 0);
+    bodies uniform over sphere/cube/ellipsoid with ellipsoid axis ratios
+    U[0.5, 2] on y and z; Re log-uniform; levels uniform; target active
+    cells log-uniform, reached through the resolution (seed 1)."""
+    ra = np.random.default_rng(seed_arrivals)
+    rb = np.random.default_rng(seed_bodies)
+    gaps = ra.exponential(1.0 / rate, size=n) if rate > 0 else np.zeros(n)
+    gaps[0] = 0.0
+    arrivals = np.cumsum(gaps)
+    todo = []
+    for i in range(n):
+        body = ["sphere", "cube", "ellipsoid"][int(rb.integers(0, 3))]
+        axes = (1.0, float(rb.uniform(0.5, 2.0)), float(rb.uniform(0.5, 2.0)))
+        re = float(np.exp(rb.uniform(np.log(reynolds[0]), np.log(reynolds[1]))))
+        lv = int(rb.integers(levels[0], levels[1] + 1))
+        target = float(np.exp(rb.uniform(np.log(cells[0]), np.log(cells[1]))))
+        req = Request(rid=i, body=body, axes=axes, reynolds=re, levels=lv, finest_steps=finest_steps,
+                      arrival=float(arrivals[i]))
+        todo.append((req, target))
+    with ThreadPoolExecutor(max_workers=8) as pool:
+        return list(pool.map(lambda rt: size_request(*rt), todo))
+
+
+def _lu(grids, coarse_steps: int) -> int:
+    return sum(g.n_cells * 2 ** (g.level_id - 1) for g in grids) * coarse_steps
+
+
+def device_memory() -> tuple:
+    """(free, total) bytes of the current HIP device."""
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return free.value, total.value
+
+
+def _wait_until(t0: float, t: float) -> None:
+    d = t0 + t - time.perf_counter()
+    if d > 0:
+        time.sleep(d)
+
+
+def run_sequential(requests: list, storage: int = N.LB_STORAGE_FP32, strict_fp: bool = False) -> list:
+    """One case at a time, in arrival order (run_all_cases)."""
+    t0 = time.perf_counter()
+    out = []
+    for req in sorted(requests, key=lambda r: r.arrival):
+        _wait_until(t0, req.arrival)
+        start = time.perf_counter() - t0
+        cfg, mesh = request_case(req)
+        grids, params, mesh = build_case(cfg, mesh)
+        n = req.coarse_steps(len(grids))
+        s = Solver(cfg, grids, params, mesh, strict_fp=strict_fp, storage=storage)
+        s.init_equilibrium()
+        s.run(n, diag_freq=10 ** 9)
+        f = s.compute_aerodynamics()
+        nbytes = s.device_bytes
+        s.close()
+        out.append(Result(req.rid, req.arrival, start, time.perf_counter() - t0, len(grids),
+                          sum(g.n_cells for g in grids), n, _lu(grids, n), f["Cd"], f["Cl"], nbytes))
+    return out
+
+
+@dataclass
+class _Prepared:
+    req: Request
+    solver: Solver
+    grids: list
+    n_steps: int
+    nbytes: int
+    start: float
+
+
+@dataclass
+class _Budget:
+    limit: int
+    used: int = 0
+    cv: threading.Condition = field(default_factory=threading.Condition)
+
+    def acquire(self, n: int) -> None:
+        with self.cv:
+            if n > self.limit:
+                raise MemoryError(f"case needs {n} bytes, budget is {self.limit}")
+            while self.used + n > self.limit:
+                self.cv.wait()
+            self.used += n
+
+    def release(self, n: int) -> None:
+        with self.cv:
+            self.used -= n
+            self.cv.notify_all()
+
+
+def run_batched(requests: list, storage: int = N.LB_STORAGE_FP16, strict_fp: bool = False,
+                hbm_budget: int | None = None, reserve: int = 8 << 30, setup_workers: int = 4,
+                chunk: int = 4, graph: bool = True, schedule: int = N.LB_SCHED_XCD) -> list:
+    """Continuous batching over one Engine (see the module docstring)."""
+    if hbm_budget is None:
+        free, _total = device_memory()
+        hbm_budget = free - reserve
+    budget = _Budget(hbm_budget)
+    engine = Engine(storage=storage, strict_fp=strict_fp)
+    engine.set_options(graph=graph, schedule=schedule)
+    t0 = time.perf_counter()
+    ready: deque = deque()
+    lock = threading.Lock()
+    errors: list = []
+
+    def prepare(req: Request) -> None:
+        try:
+            start = time.perf_counter() - t0
+            cfg, mesh = request_case(req)
+            grids, params, mesh = build_case(cfg, mesh)
+            n = req.coarse_steps(len(grids))
+            nbytes = footprint(cfg, grids, params, mesh, storage, len(ramp_table(cfg, n)))
+            budget.acquire(nbytes)
+            s = Solver(cfg, grids, params, mesh, strict_fp=strict_fp, storage=storage)
+            s.init_equilibrium()
+            with lock:
+                ready.append(_Prepared(req, s, grids, n, nbytes, start))
+        except BaseException as exc:  # surfaced by the serving loop
+            with lock:
+                errors.append(exc)
+
+    pending = sorted(requests, key=lambda r: r.arrival)
+    active: dict = {}
+    out = []
+    with ThreadPoolExecutor(max_workers=setup_workers) as pool:
+        while len(out) < len(requests):
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                pool.submit(prepare, pending.pop(0))
+            with lock:
+                if errors:
+                    raise errors[0]
+                while ready:
+                    p = ready.popleft()
+                    engine.add(p.solver, p.n_steps)
+                    active[p.solver] = p
+            if active:
+                engine.step(chunk)
+                engine.synchronize()
+                for s in [s for s in active if engine.done(s)]:
+                    p = active.pop(s)
+                    engine.remove(s)
+                    f = s.compute_aerodynamics()
+                    nbytes = s.device_bytes
+                    s.close()
+                    budget.release(p.nbytes)
+                    out.append(Result(p.req.rid, p.req.arrival, p.start, time.perf_counter() - t0,
+                                      len(p.grids), sum(g.n_cells for g in p.grids), p.n_steps,
+                                      _lu(p.grids, p.n_steps), f["Cd"], f["Cl"], nbytes))
+            else:
+                nxt = pending[0].arrival - now if pending else 0.002
+                time.sleep(min(max(nxt, 0.0), 0.002))
+    engine.close()
+    return sorted(out, key=lambda r: r.rid)
+
+
+def summary(results: list) -> dict:
+    lat = np.array([r.latency for r in results])
+    span = max(r.finish for r in results) - min(r.arrival for r in results)
+    lu = sum(r.lu for r in results)
+    return {"n": len(results), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "span_s": float(span), "cases_per_hour": len(results) / span * 3600.0,
+            "mlups_true": lu / span / 1e6}

@@ -108,6 +108,33 @@ class ForceAccumulator:
         self.n_samples += 1
 
 
+def lb_config(cfg: CaseConfig, grids: list, params: DomainParameters, storage: int, strict_fp: bool) -> N.LbConfig:
+    c = N.LbConfig()
+    c.n_levels = len(grids)
+    c.domain_nx, c.domain_ny, c.domain_nz = params.nx_coarse, params.ny_coarse, params.nz_coarse
+    c.temporal_interpolation = int(cfg.temporal_interpolation)
+    c.ddf_enabled = int(cfg.ddf_enabled)
+    c.q_min_threshold = cfg.q_min_threshold
+    c.storage = storage
+    c.strict_fp = int(strict_fp)
+    return c
+
+
+def footprint(cfg: CaseConfig, grids: list, params: DomainParameters, mesh: SolverMesh | None,
+              storage: int = N.LB_STORAGE_FP32, n_ramp_batches: int = 0) -> int:
+    """Exact device bytes of a case (lb_footprint), computed without a GPU."""
+    lib = N.hip_lib()
+    keep: list = []
+    descs = (N.LbLevelDesc * len(grids))(*[level_desc(g, keep) for g in grids])
+    n_tri = mesh.n if (mesh is not None and cfg.forces_enabled) else 0
+    out = C.c_int64()
+    rc = lib.lb_footprint(C.byref(lb_config(cfg, grids, params, storage, False)), descs, len(grids), n_tri,
+                          int(n_ramp_batches), C.byref(out))
+    if rc != 0:
+        raise LudwigError(f"lb_footprint failed ({rc})")
+    return out.value
+
+
 class Solver:
     def __init__(self, cfg: CaseConfig, grids: list, params: DomainParameters, mesh: SolverMesh | None = None,
                  device: int = 0, strict_fp: bool = False, storage: int = N.LB_STORAGE_FP32):
@@ -117,14 +144,8 @@ class Solver:
         self.mesh = mesh
         self.lib = N.hip_lib()
         self._ctx = C.c_void_p()
-        c = N.LbConfig()
-        c.n_levels = len(grids)
-        c.domain_nx, c.domain_ny, c.domain_nz = params.nx_coarse, params.ny_coarse, params.nz_coarse
-        c.temporal_interpolation = int(cfg.temporal_interpolation)
-        c.ddf_enabled = int(cfg.ddf_enabled)
-        c.q_min_threshold = cfg.q_min_threshold
-        c.storage = storage
-        c.strict_fp = int(strict_fp)
+        self.engine = None  # set while attached to a batched Engine
+        c = lb_config(cfg, grids, params, storage, strict_fp)
         rc = self.lib.lb_create(C.byref(self._ctx), device, C.byref(c))
         if rc != 0:
             raise LudwigError(f"lb_create failed ({rc})")
@@ -144,6 +165,8 @@ class Solver:
     # -- lifecycle -------------------------------------------------------
     def close(self) -> None:
         if self._ctx:
+            if self.engine is not None:
+                self.engine.remove(self)
             self.lib.lb_destroy(self._ctx)
             self._ctx = C.c_void_p()
 

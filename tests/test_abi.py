@@ -50,3 +50,19 @@ def test_error_paths_without_device():
     assert lib.lb_step_batch(None, 1, 1, 0.0, None) == -1
     assert lib.lb_synchronize(None) == -1
     assert lib.lb_device_bytes(None) == 0
+
+
+def test_footprint_plans_without_gpu():
+    """lb_footprint runs the upload code in counting mode: no device needed,
+    FP16 storage needs less than FP32, and the count grows with the mesh."""
+    from openludwig_amd import _native as N
+    from openludwig_amd import cases
+    from openludwig_amd.solver import build_case, footprint
+    cfg, mesh = cases.tiny_sphere(8, 2)
+    grids, params, mesh = build_case(cfg, mesh)
+    b32 = footprint(cfg, grids, params, mesh, N.LB_STORAGE_FP32)
+    b16 = footprint(cfg, grids, params, mesh, N.LB_STORAGE_FP16)
+    n = sum(g.n_cells for g in grids)
+    assert 216 * n < b32 < 400 * n
+    assert 108 * n < b16 < b32
+    assert footprint(cfg, grids, params, mesh, N.LB_STORAGE_FP16, 10) == b16 + 40
