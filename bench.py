@@ -50,6 +50,8 @@ def main() -> int:
     ap.add_argument("--storage", default="fp32", choices=sorted(B_ALG_WMLES),
                     help="population storage (fp16/bf16: offsets f - w, FP32 arithmetic)")
     ap.add_argument("--cpu-steps", type=int, default=1, help="coarse steps timed for the CPU baseline (0: skip)")
+    ap.add_argument("--serving-requests", type=int, default=8,
+                    help="requests in the serving comparison, FP16 batched vs plain FP32 (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU form of this "
@@ -150,6 +152,13 @@ def main() -> int:
                          f"updates), C oracle (FP32 mirror, OpenMP)", "seconds": round(tcpu, 2)}
         log(f"cpu baseline {c_true:.1f} MLUPS on {threads} threads ({tcpu:.1f}s)")
 
+    serving = None
+    if args.serving_requests > 0:
+        from openludwig_amd import serving as S
+        if "solver" in locals():
+            del solver
+        serving = S.compare(args.serving_requests, log=log)
+
     line = {
         "metric": "MLUPS & % HBM roofline; served cases/h + p99 latency vs plain FP32",
         "value": round(m_true, 1),
@@ -176,6 +185,7 @@ def main() -> int:
                      "kernel": "collide_kernel (finest level)", "bytes_per_launch": bytes_launch,
                      "avg_launch_ms": round(avg_ms, 4)},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -312,3 +312,64 @@ def summary(results: list) -> dict:
     return {"n": len(results), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "span_s": float(span), "cases_per_hour": len(results) / span * 3600.0,
             "mlups_true": lu / span / 1e6}
+
+
+def compare(n: int = 8, cells: tuple = (5e5, 4e6), finest_steps: int = 1024, load: float = 1.5,
+            setup_workers: int = 4, log=print) -> dict:
+    """The serving benchmark: plain FP32 sequential against FP16 continuous
+    batching on the same requests.  Capacity: cases/h of each form with all
+    requests present at t = 0 (closed runs).  Latency: p50/p99 of both forms
+    under Poisson arrivals at `load` times the plain form's capacity."""
+    reqs = make_requests(n, rate=0.0, cells=cells, finest_steps=finest_steps)
+    closed = run_sequential(reqs, storage=N.LB_STORAGE_FP32)
+    cs = summary(closed)
+    mu = n / cs["span_s"]
+    log(f"serving: plain FP32 closed run {cs['span_s']:.1f}s for {n} cases ({cs['cases_per_hour']:.0f} cases/h)")
+    closed_b = run_batched(reqs, storage=N.LB_STORAGE_FP16, setup_workers=setup_workers)
+    cb = summary(closed_b)
+    log(f"serving: FP16 batched closed run {cb['span_s']:.1f}s ({cb['cases_per_hour']:.0f} cases/h)")
+    gaps = np.random.default_rng(0).exponential(1.0 / (load * mu), size=n)
+    gaps[0] = 0.0
+    for r, t in zip(reqs, np.cumsum(gaps)):
+        r.arrival = float(t)
+    seq = run_sequential(reqs, storage=N.LB_STORAGE_FP32)
+    bat = run_batched(reqs, storage=N.LB_STORAGE_FP16, setup_workers=setup_workers)
+    ss, bs = summary(seq), summary(bat)
+    dcd = [abs(a.cd - b.cd) / max(abs(a.cd), 1e-12) for a, b in zip(seq, bat)]
+    log(f"serving: sequential FP32 p50 {ss['p50_s']:.2f}s p99 {ss['p99_s']:.2f}s {ss['cases_per_hour']:.0f} cases/h; "
+        f"batched FP16 p50 {bs['p50_s']:.2f}s p99 {bs['p99_s']:.2f}s {bs['cases_per_hour']:.0f} cases/h")
+    return {"requests": n, "cells_range": list(cells), "finest_steps": finest_steps,
+            "capacity_cases_per_hour": {"plain_fp32_sequential": round(cs["cases_per_hour"], 1),
+                                        "fp16_batched": round(cb["cases_per_hour"], 1),
+                                        "ratio": round(cb["cases_per_hour"] / cs["cases_per_hour"], 3)},
+            "arrival_rate_per_s": round(load * mu, 4), "load_vs_plain_throughput": load,
+            "plain_fp32_sequential": {k: round(v, 3) for k, v in ss.items()},
+            "fp16_batched": {k: round(v, 3) for k, v in bs.items()},
+            "cases_per_hour_ratio": round(bs["cases_per_hour"] / ss["cases_per_hour"], 3),
+            "p99_ratio": round(ss["p99_s"] / bs["p99_s"], 3),
+            "cd_rel_diff_fp16_vs_fp32": {"max": float(max(dcd)), "median": float(np.median(dcd))},
+            "per_request": [{"rid": a.rid, "cells": a.cells, "levels": a.levels, "coarse_steps": a.coarse_steps,
+                             "cd_fp32": round(a.cd, 6), "cd_fp16": round(b.cd, 6),
+                             "latency_fp32_s": round(a.latency, 3), "latency_fp16_s": round(b.latency, 3)}
+                            for a, b in zip(seq, bat)]}
+
+
+def main() -> int:
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="serving benchmark: FP16 continuous batching vs plain FP32")
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--min-cells", type=float, default=1e6)
+    ap.add_argument("--max-cells", type=float, default=40e6)
+    ap.add_argument("--finest-steps", type=int, default=1024)
+    ap.add_argument("--load", type=float, default=1.5)
+    ap.add_argument("--workers", type=int, default=4)
+    a = ap.parse_args()
+    res = compare(a.requests, (a.min_cells, a.max_cells), a.finest_steps, a.load, a.workers,
+                  log=lambda m: print(m, flush=True))
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
