@@ -82,16 +82,8 @@ def main() -> int:
             solver.execute_timestep_batch(t, be - t + 1, ramp_velocity(cfg.u_lattice, be, cfg.ramp_steps))
             t = be + 1
 
-    try:
-        import torch
-        has_torch = torch.cuda.is_available()
-    except Exception:
-        has_torch = False
-
     def sync() -> None:
         solver.synchronize()
-        if has_torch:
-            torch.cuda.synchronize()
 
     run(1, args.warmup)
     sync()

@@ -200,6 +200,9 @@ int64_t lb_device_bytes(const lb_ctx* ctx);
 int lb_footprint(const lb_config* cfg, const lb_level_desc* levels, int32_t n_levels, int64_t n_tri,
                  int32_t n_ramp_batches, int64_t* bytes);
 
+/* Free and total device memory (hipMemGetInfo), for the admission budget. */
+int lb_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
+
 /* Kernel timing (no counterpart in the reference, which only times the host
  * loop, main.jl:232): when enabled, HIP events are recorded on the context's
  * stream around every stream-collide launch.  lb_profile_read synchronises,
@@ -246,11 +249,12 @@ int lb_engine_synchronize(lb_engine* e);
 int lb_engine_case_progress(const lb_engine* e, const lb_ctx* ctx, int64_t* t_done);
 
 /* Launch options (picked by the autotuner; results do not depend on them):
- * graph (1: replay a captured hipGraph per step) and the work-group
- * schedule of the stream-collide launches (LB_SCHED_*).                    */
+ * graph (1: replay a captured hipGraph per step), the work-group schedule
+ * of the stream-collide launches (LB_SCHED_*) and the occupancy of the
+ * interior-block stream-collide kernel, 5 or 6 waves per SIMD.            */
 #define LB_SCHED_XCD 0    /* work-groups dealt to XCDs in contiguous runs */
 #define LB_SCHED_LINEAR 1 /* work-group i takes half-tile i               */
-int lb_engine_set_options(lb_engine* e, int32_t use_graph, int32_t schedule);
+int lb_engine_set_options(lb_engine* e, int32_t use_graph, int32_t schedule, int32_t waves);
 
 /* HBM bandwidth probe (no counterpart in the reference): streams `bytes`
  * with 16-byte accesses; gbps_out[0] copy (read + write counted),

@@ -92,9 +92,10 @@ HIP_EXPORTS = {
     "lb_engine_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "lb_engine_synchronize": (C.c_int, [C.c_void_p]),
     "lb_engine_case_progress": (C.c_int, [C.c_void_p, C.c_void_p, i64p]),
-    "lb_engine_set_options": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "lb_engine_set_options": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "lb_footprint": (C.c_int, [C.POINTER(LbConfig), C.POINTER(LbLevelDesc), C.c_int32, C.c_int64, C.c_int32,
                                i64p]),
+    "lb_device_memory": (C.c_int, [C.c_int, i64p, i64p]),
 }
 
 LB_SCHED_XCD = 0

@@ -38,7 +38,7 @@ constexpr int Q = 27;
 
 struct Kern {
     hipError_t (*collide)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
-                          int, hipStream_t);
+                          int, int, int, hipStream_t);
     hipError_t (*thermal)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
                           hipStream_t);
     hipError_t (*ghost_fill)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
@@ -102,6 +102,7 @@ struct Level {
     lbk::Work* wlk = nullptr;
     int cur = 0;        // buffer index written by the last step (0 before any step)
     std::vector<int32_t> order_host;  // processing order (engine work lists)
+    int32_t n_int = 0;                // interior blocks: the first n_int of the order
     // kernel timing: event pairs recorded around collide launches
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     size_t ev_used = 0;
@@ -147,8 +148,9 @@ struct EngineCase {
 
 // Work lists of one depth of the engine's recursion (concatenated over cases).
 struct DepthWork {
-    lbk::Work *col = nullptr, *gh = nullptr, *lk = nullptr, *tcol = nullptr, *tgh = nullptr, *tlk = nullptr;
-    int32_t ncol = 0, ngh = 0, nlk = 0, ntcol = 0, ntgh = 0, ntlk = 0;
+    lbk::Work *col = nullptr, *colb = nullptr, *gh = nullptr, *lk = nullptr, *tcol = nullptr, *tgh = nullptr,
+              *tlk = nullptr;
+    int32_t ncol = 0, ncolb = 0, ngh = 0, nlk = 0, ntcol = 0, ntgh = 0, ntlk = 0;
     int wide = 0, wide_g = 0;
 };
 
@@ -170,6 +172,7 @@ struct lb_engine {
     // options
     bool use_graph = true;
     int sched = 0;
+    int waves = 5;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -413,7 +416,9 @@ int step_level(lb_ctx* c, int li, int64_t j) {
             e1 = L.ev[L.ev_used].second;
             L.ev_used++;
         }
-        HIPCHK(c, c->k->collide(las, st, L.wcol, L.nb, j, storage, L.wide ? 1 : 0, 0, c->stream));
+        HIPCHK(c, c->k->collide(las, st, L.wcol, L.n_int, j, storage, L.wide ? 1 : 0, 0, 0, 5, c->stream));
+        HIPCHK(c, c->k->collide(las, st, L.wcol + L.n_int, L.nb - L.n_int, j, storage, L.wide ? 1 : 0, 1, 0, 5,
+                                c->stream));
         if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
         if (L.n_lchunks > 0) HIPCHK(c, c->k->bouzidi(las, st, L.wlk, L.n_lchunks, j, storage, 0, c->stream));
         if (ddf) {
@@ -603,7 +608,7 @@ int engine_rebuild(lb_engine* e) {
     RC(eupload(e, &e->states, sts));
     e->depth.assign((size_t)e->lmax, DepthWork());
     for (int d = 1; d <= e->lmax; d++) {
-        std::vector<lbk::Work> col, gh, lk, tcol, tgh, tlk;
+        std::vector<lbk::Work> col, colb, gh, lk, tcol, tgh, tlk;
         DepthWork& W = e->depth[(size_t)d - 1];
         for (size_t i = 0; i < e->cases.size(); i++) {
             const lb_ctx* c = e->cases[i].c;
@@ -614,9 +619,11 @@ int engine_rebuild(lb_engine* e) {
             const int li = base[i] + l - 1, ci = (int)i;
             if (lev.nb == 0) continue;
             const bool ddf = c->cfg.ddf_enabled && lev.g[0] != nullptr;
-            for (int32_t b : lev.order_host) {
+            for (size_t q = 0; q < lev.order_host.size(); q++) {
+                const int32_t b = lev.order_host[q];
                 const lbk::Work w = mkwork(li, b, ci, lev.nbr + (size_t)b * Q);
-                col.push_back(w);
+                if ((int32_t)q < lev.n_int) col.push_back(w);
+                else colb.push_back(w);
                 if (ddf) tcol.push_back(w);
             }
             if (l > 1)
@@ -632,12 +639,14 @@ int engine_rebuild(lb_engine* e) {
             if (lev.wide_g) W.wide_g = 1;
         }
         RC(eupload(e, &W.col, col));
+        RC(eupload(e, &W.colb, colb));
         RC(eupload(e, &W.gh, gh));
         RC(eupload(e, &W.lk, lk));
         RC(eupload(e, &W.tcol, tcol));
         RC(eupload(e, &W.tgh, tgh));
         RC(eupload(e, &W.tlk, tlk));
         W.ncol = (int32_t)col.size();
+        W.ncolb = (int32_t)colb.size();
         W.ngh = (int32_t)gh.size();
         W.nlk = (int32_t)lk.size();
         W.ntcol = (int32_t)tcol.size();
@@ -661,7 +670,9 @@ hipError_t engine_op(lb_engine* e, int d, int64_t j) {
     } while (0)
     const lbk::CaseState* st = e->states;
     if (W.ngh) ETRY(k->ghost_fill(e->las, st, W.gh, W.ngh, j, e->storage, 0, e->stream));
-    if (W.ncol) ETRY(k->collide(e->las, st, W.col, W.ncol, j, e->storage, W.wide, e->sched, e->stream));
+    if (W.ncol) ETRY(k->collide(e->las, st, W.col, W.ncol, j, e->storage, W.wide, 0, e->sched, e->waves, e->stream));
+    if (W.ncolb)
+        ETRY(k->collide(e->las, st, W.colb, W.ncolb, j, e->storage, W.wide, 1, e->sched, e->waves, e->stream));
     if (W.nlk) ETRY(k->bouzidi(e->las, st, W.lk, W.nlk, j, e->storage, 0, e->stream));
     if (W.ntcol) {
         if (W.ntgh) ETRY(k->ghost_fill(e->las, st, W.tgh, W.ntgh, j, e->storage, 1, e->stream));
@@ -865,8 +876,24 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
             key[(size_t)b] = {m, b};
         }
         std::sort(key.begin(), key.end());
-        std::vector<int32_t> order((size_t)nb);
-        for (int32_t i = 0; i < nb; i++) order[(size_t)i] = key[(size_t)i].second;
+        // processing order: Morton order of the interior blocks (all 26
+        // neighbours present, ghosts included), then of the boundary blocks
+        std::vector<int32_t> order;
+        order.reserve((size_t)nb);
+        for (int pass = 0; pass < 2; pass++)
+            for (int32_t i = 0; i < nb; i++) {
+                const int32_t b = key[(size_t)i].second;
+                bool interior = true;
+                for (int k = 0; k < Q; k++) interior = interior && nbr[(size_t)b * Q + k] >= 0;
+                if (interior == (pass == 0)) order.push_back(b);
+            }
+        L.n_int = 0;
+        for (int32_t b : order) {
+            bool interior = true;
+            for (int k = 0; k < Q; k++) interior = interior && nbr[(size_t)b * Q + k] >= 0;
+            if (!interior) break;
+            L.n_int++;
+        }
         RC(h2d(c, L.order, order.data(), order.size()));
         L.order_host = order;
         std::vector<lbk::Work> wcol((size_t)nb);
@@ -1287,12 +1314,13 @@ int lb_engine_case_progress(const lb_engine* e, const lb_ctx* c, int64_t* t_done
     return LB_EINVAL;
 }
 
-int lb_engine_set_options(lb_engine* e, int32_t use_graph, int32_t schedule) {
-    if (!e || (schedule != LB_SCHED_XCD && schedule != LB_SCHED_LINEAR)) return LB_EINVAL;
+int lb_engine_set_options(lb_engine* e, int32_t use_graph, int32_t schedule, int32_t waves) {
+    if (!e || (schedule != LB_SCHED_XCD && schedule != LB_SCHED_LINEAR) || waves < 5 || waves > 6) return LB_EINVAL;
     EHIPCHK(e, hipSetDevice(e->device));
     EHIPCHK(e, hipStreamSynchronize(e->stream));
     e->use_graph = use_graph != 0;
     e->sched = schedule;
+    e->waves = waves;
     // the captured graph bakes the schedule in
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
@@ -1322,6 +1350,16 @@ int lb_footprint(const lb_config* cfg, const lb_level_desc* levels, int32_t n_le
         RC(dalloc(&c, &u, (size_t)n_ramp_batches));
     }
     *bytes = c.bytes;
+    return LB_OK;
+}
+
+int lb_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return LB_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return LB_EHIP;
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return LB_EHIP;
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
     return LB_OK;
 }
 

@@ -133,8 +133,9 @@ struct ForceArgs {
 
 #define LBK_DECLARE(NS)                                                                                  \
     namespace NS {                                                                                       \
-    hipError_t collide(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work, int32_t n_tiles, int64_t j, int storage, \
-                       int wide, int sched, hipStream_t s);                                              \
+    hipError_t collide(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work,    \
+                       int32_t n_tiles, int64_t j, int storage, int wide, int boundary, int sched,       \
+                       int waves, hipStream_t s);                                                        \
     hipError_t thermal(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work, int32_t n_tiles, int64_t j,      \
                        int wide, int sched, hipStream_t s);                                              \
     hipError_t ghost_fill(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work, int32_t n_chunks, int64_t j,  \

@@ -486,12 +486,17 @@ __device__ __forceinline__ const GF* gaddr(const void* base, OffT off) {
 // A/B buffers chosen by the caller and no f_post_collision copy (Bouzidi
 // reads f_out before it corrects it, see bouzidi_gather below).
 // Occupancy: the collision needs ~160 VGPRs, which allows 3 waves per SIMD.
-// Capping the kernel at 96 VGPRs (5 waves per SIMD; 5 work-groups of 28 KB
-// LDS per CU) spills ~50 values per lane to scratch, which stays in cache,
-// and hides more of the pull's memory latency: measured 20 % faster with
-// FP16 storage and equal with FP32 on the 52.5M-cell cube case.
-template <int ST, class OffT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5)))
+// Capping the kernel's VGPRs (W waves per SIMD) spills values to scratch,
+// which stays in cache, and lets more waves hide the pull's memory latency.
+// Boundary blocks (some neighbour outside the domain) need the 27 KB cascade
+// buffer in LDS, which limits them to 5 work-groups per CU; they run at 5.
+// Interior blocks need no cascade buffer; W = 5 or 6 is a launch option
+// (picked by the autotuner).  The two compile to the same floating-point
+// operations (tested bit-identical); a 4-wave build contracts differently and
+// is not offered.  Measured on the 52.5M-cell cube case: FP16 best at 5,
+// FP32 equal at 4 to 6.
+template <int ST, class OffT, bool BND, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 collide_kernel(const LevelArgs* __restrict__ las,
                                                       const CaseState* __restrict__ states,
                                                       const Work* __restrict__ work, int32_t n_tiles, int64_t j,
@@ -499,7 +504,7 @@ collide_kernel(const LevelArgs* __restrict__ las,
     __shared__ OffT ptab[108];
     __shared__ OffT vtab[32];
     __shared__ uint64_t smask[2];
-    __shared__ float sfix[27 * 256];
+    __shared__ float sfix[BND ? 27 * 256 : 1];
     Work wk;
     int half;
     if (!schedule(work, n_tiles, sched, wk, half)) return;
@@ -555,30 +560,33 @@ collide_kernel(const LevelArgs* __restrict__ las,
         }
     }
     // lanes whose pull crosses into a missing block (domain boundary)
-    uint32_t miss = 0u;
-    const uint32_t m = wave_miss(smask, wv);
-    if (m != 0u) {
-#pragma unroll
-        for (int k = 0; k < 27; k++) {
-            const int cx = CX(k), cy = CY(k), cz = CZ(k);
-            const int ox = cx == 1 ? oxp : (cx == -1 ? oxm : 0);
-            const int oy = cy == 1 ? oyp : (cy == -1 ? oym : 0);
-            miss |= ((m >> (9 * (cz + 1) + (ox + 1) + 3 * (oy + 1))) & 1u) << k;
-        }
-    }
     const float sp = gp(a.sponge)[i4];
     const float wdist = a.wall_model == 1 ? gp(a.wall_dist)[i4] : 0.0f;
-    // Boundary cascade for sources in no block: one inlined copy of the
-    // cascade in a rolled loop over k, results passed through LDS (a
-    // register array cannot be indexed by a run-time k without spilling).
-    if (miss != 0u) {
-#pragma unroll 1
-        for (int k = 0; k < 27; k++)
-            if (miss & (1u << k))
-                sfix[k * 256 + threadIdx.x] = bc_value<ST>(a, k, b, c, gx - CX(k), gy - CY(k), gz - CZ(k), gy, gz);
+    if constexpr (BND) {
+        uint32_t miss = 0u;
+        const uint32_t m = wave_miss(smask, wv);
+        if (m != 0u) {
 #pragma unroll
-        for (int k = 0; k < 27; k++)
-            if (miss & (1u << k)) A[k] = sfix[k * 256 + threadIdx.x];
+            for (int k = 0; k < 27; k++) {
+                const int cx = CX(k), cy = CY(k), cz = CZ(k);
+                const int ox = cx == 1 ? oxp : (cx == -1 ? oxm : 0);
+                const int oy = cy == 1 ? oyp : (cy == -1 ? oym : 0);
+                miss |= ((m >> (9 * (cz + 1) + (ox + 1) + 3 * (oy + 1))) & 1u) << k;
+            }
+        }
+        // Boundary cascade for sources in no block: one inlined copy of the
+        // cascade in a rolled loop over k, results passed through LDS (a
+        // register array cannot be indexed by a run-time k without spilling).
+        if (miss != 0u) {
+#pragma unroll 1
+            for (int k = 0; k < 27; k++)
+                if (miss & (1u << k))
+                    sfix[k * 256 + threadIdx.x] =
+                        bc_value<ST>(a, k, b, c, gx - CX(k), gy - CY(k), gz - CZ(k), gy, gz);
+#pragma unroll
+            for (int k = 0; k < 27; k++)
+                if (miss & (1u << k)) A[k] = sfix[k * 256 + threadIdx.x];
+        }
     }
     float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
 #pragma unroll
@@ -878,18 +886,32 @@ collide_kernel(const LevelArgs* __restrict__ las,
 
 inline unsigned padded_groups(int nb) { return ((unsigned)nb * 2u + 7u) & ~7u; }
 
-hipError_t collide(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_tiles, int64_t j, int storage, int wide,
-                   int sched, hipStream_t s) {
-    if (n_tiles <= 0) return hipSuccess;
+template <int ST, class OffT>
+void collide_launch(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_tiles, int64_t j,
+                    int boundary, int sched, int waves, hipStream_t s) {
     const dim3 g(padded_groups(n_tiles)), t(256);
+    if (boundary)
+        hipLaunchKernelGGL((collide_kernel<ST, OffT, true, 5>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+    else if (waves == 6)
+        hipLaunchKernelGGL((collide_kernel<ST, OffT, false, 6>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+    else
+        hipLaunchKernelGGL((collide_kernel<ST, OffT, false, 5>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+}
+
+// boundary = 0 requires every block of the work list to have all 26
+// neighbours (blocks or ghost blocks).  waves: occupancy of the interior
+// kernel (5 or 6 waves per SIMD).
+hipError_t collide(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_tiles, int64_t j,
+                   int storage, int wide, int boundary, int sched, int waves, hipStream_t s) {
+    if (n_tiles <= 0) return hipSuccess;
     if (wide) {
-        if (storage == 1) hipLaunchKernelGGL((collide_kernel<1, uint64_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
-        else if (storage == 2) hipLaunchKernelGGL((collide_kernel<2, uint64_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
-        else hipLaunchKernelGGL((collide_kernel<0, uint64_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+        if (storage == 1) collide_launch<1, uint64_t>(las, states, work, n_tiles, j, boundary, sched, waves, s);
+        else if (storage == 2) collide_launch<2, uint64_t>(las, states, work, n_tiles, j, boundary, sched, waves, s);
+        else collide_launch<0, uint64_t>(las, states, work, n_tiles, j, boundary, sched, waves, s);
     } else {
-        if (storage == 1) hipLaunchKernelGGL((collide_kernel<1, uint32_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
-        else if (storage == 2) hipLaunchKernelGGL((collide_kernel<2, uint32_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
-        else hipLaunchKernelGGL((collide_kernel<0, uint32_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+        if (storage == 1) collide_launch<1, uint32_t>(las, states, work, n_tiles, j, boundary, sched, waves, s);
+        else if (storage == 2) collide_launch<2, uint32_t>(las, states, work, n_tiles, j, boundary, sched, waves, s);
+        else collide_launch<0, uint32_t>(las, states, work, n_tiles, j, boundary, sched, waves, s);
     }
     return hipGetLastError();
 }

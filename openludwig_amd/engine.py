@@ -37,6 +37,7 @@ class Engine:
             raise LudwigError(f"lb_engine_create failed ({rc})")
         self.storage = storage
         self.cases: dict = {}
+        self.options = {"graph": True, "schedule": N.LB_SCHED_XCD, "waves": 5}
 
     def close(self) -> None:
         if self._e:
@@ -85,8 +86,35 @@ class Engine:
     def done(self, solver: Solver) -> bool:
         return self.progress(solver) >= self.cases[solver]
 
-    def set_options(self, graph: bool = True, schedule: int = N.LB_SCHED_XCD) -> None:
-        self._check(self.lib.lb_engine_set_options(self._e, int(graph), int(schedule)), "lb_engine_set_options")
+    def set_options(self, graph: bool = True, schedule: int = N.LB_SCHED_XCD, waves: int = 5) -> None:
+        self._check(self.lib.lb_engine_set_options(self._e, int(graph), int(schedule), int(waves)),
+                    "lb_engine_set_options")
+        self.options = {"graph": bool(graph), "schedule": int(schedule), "waves": int(waves)}
+
+    def autotune(self, trial_steps: int = 2, candidates=None) -> dict:
+        """Pick the launch options (graph replay, work-group schedule,
+        interior-kernel occupancy) by timing each on the attached cases.
+        Every option computes the same results, so the trial steps are real
+        steps of the cases.  Returns the chosen options and the timings."""
+        import itertools
+        import time
+        if candidates is None:
+            candidates = [dict(graph=g, schedule=sc, waves=w) for g, sc, w in
+                          itertools.product((True, False), (N.LB_SCHED_XCD, N.LB_SCHED_LINEAR), (5, 6))]
+        timings = []
+        for opt in candidates:
+            if not any(not self.done(s) for s in self.cases):
+                break
+            self.set_options(**opt)
+            self.step(1)  # graph capture / warm-up
+            self.synchronize()
+            t = time.perf_counter()
+            self.step(trial_steps)
+            self.synchronize()
+            timings.append(((time.perf_counter() - t) / trial_steps, opt))
+        best = min(timings, key=lambda x: x[0])[1] if timings else self.options
+        self.set_options(**best)
+        return {"options": best, "trials": [{"s_per_step": round(t, 6), **o} for t, o in timings]}
 
     def run_to_completion(self) -> None:
         """Step until every attached case has done its coarse steps."""

@@ -173,12 +173,11 @@ def _lu(grids, coarse_steps: int) -> int:
     return sum(g.n_cells * 2 ** (g.level_id - 1) for g in grids) * coarse_steps
 
 
-def device_memory() -> tuple:
-    """(free, total) bytes of the current HIP device."""
-    hip = C.CDLL("libamdhip64.so")
-    free, total = C.c_size_t(), C.c_size_t()
-    if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
-        raise RuntimeError("hipMemGetInfo failed")
+def device_memory(device: int = 0) -> tuple:
+    """(free, total) bytes of the HIP device."""
+    free, total = C.c_int64(), C.c_int64()
+    if N.hip_lib().lb_device_memory(device, C.byref(free), C.byref(total)) != 0:
+        raise RuntimeError("lb_device_memory failed")
     return free.value, total.value
 
 
@@ -188,7 +187,7 @@ def _wait_until(t0: float, t: float) -> None:
         time.sleep(d)
 
 
-def run_sequential(requests: list, storage: int = N.LB_STORAGE_FP32, strict_fp: bool = False) -> list:
+def run_sequential(requests: list, storage: int = N.LB_STORAGE_FP32, strict_fp: bool = False, log=None) -> list:
     """One case at a time, in arrival order (run_all_cases)."""
     t0 = time.perf_counter()
     out = []
@@ -206,6 +205,8 @@ def run_sequential(requests: list, storage: int = N.LB_STORAGE_FP32, strict_fp: 
         s.close()
         out.append(Result(req.rid, req.arrival, start, time.perf_counter() - t0, len(grids),
                           sum(g.n_cells for g in grids), n, _lu(grids, n), f["Cd"], f["Cl"], nbytes))
+        if log:
+            log(f"  sequential: request {req.rid} done at {out[-1].finish:.1f}s ({out[-1].cells / 1e6:.1f}M cells)")
     return out
 
 
@@ -239,16 +240,22 @@ class _Budget:
             self.cv.notify_all()
 
 
+_TUNED: dict = {}
+
+
 def run_batched(requests: list, storage: int = N.LB_STORAGE_FP16, strict_fp: bool = False,
                 hbm_budget: int | None = None, reserve: int = 8 << 30, setup_workers: int = 4,
-                chunk: int = 4, graph: bool = True, schedule: int = N.LB_SCHED_XCD) -> list:
-    """Continuous batching over one Engine (see the module docstring)."""
+                chunk: int = 4, options: dict | None = None, tune: bool = True, log=None) -> list:
+    """Continuous batching over one Engine (see the module docstring).
+    Launch options: `options` if given, else autotuned once per (storage,
+    deepest level count) on the first cases admitted, and reused."""
     if hbm_budget is None:
         free, _total = device_memory()
         hbm_budget = free - reserve
     budget = _Budget(hbm_budget)
     engine = Engine(storage=storage, strict_fp=strict_fp)
-    engine.set_options(graph=graph, schedule=schedule)
+    if options:
+        engine.set_options(**options)
     t0 = time.perf_counter()
     ready: deque = deque()
     lock = threading.Lock()
@@ -285,6 +292,12 @@ def run_batched(requests: list, storage: int = N.LB_STORAGE_FP16, strict_fp: boo
                     p = ready.popleft()
                     engine.add(p.solver, p.n_steps)
                     active[p.solver] = p
+            if active and not options:
+                key = (storage, max(len(p.grids) for p in active.values()))
+                if key not in _TUNED and tune:
+                    _TUNED[key] = engine.autotune()["options"]
+                if key in _TUNED and engine.options != _TUNED[key]:
+                    engine.set_options(**_TUNED[key])
             if active:
                 engine.step(chunk)
                 engine.synchronize()
@@ -298,6 +311,9 @@ def run_batched(requests: list, storage: int = N.LB_STORAGE_FP16, strict_fp: boo
                     out.append(Result(p.req.rid, p.req.arrival, p.start, time.perf_counter() - t0,
                                       len(p.grids), sum(g.n_cells for g in p.grids), p.n_steps,
                                       _lu(p.grids, p.n_steps), f["Cd"], f["Cl"], nbytes))
+                    if log:
+                        log(f"  batched: request {p.req.rid} done at {out[-1].finish:.1f}s "
+                            f"({out[-1].cells / 1e6:.1f}M cells, {len(active)} still running)")
             else:
                 nxt = pending[0].arrival - now if pending else 0.002
                 time.sleep(min(max(nxt, 0.0), 0.002))
@@ -315,25 +331,27 @@ def summary(results: list) -> dict:
 
 
 def compare(n: int = 8, cells: tuple = (5e5, 4e6), finest_steps: int = 1024, load: float = 1.5,
-            setup_workers: int = 4, log=print) -> dict:
+            setup_workers: int = 4, log=print, verbose: bool = False) -> dict:
     """The serving benchmark: plain FP32 sequential against FP16 continuous
     batching on the same requests.  Capacity: cases/h of each form with all
     requests present at t = 0 (closed runs).  Latency: p50/p99 of both forms
     under Poisson arrivals at `load` times the plain form's capacity."""
     reqs = make_requests(n, rate=0.0, cells=cells, finest_steps=finest_steps)
-    closed = run_sequential(reqs, storage=N.LB_STORAGE_FP32)
+    log(f"serving: {n} requests, {sum(r.cells for r in reqs) / 1e6:.0f}M cells in total")
+    plog = log if verbose else None
+    closed = run_sequential(reqs, storage=N.LB_STORAGE_FP32, log=plog)
     cs = summary(closed)
     mu = n / cs["span_s"]
     log(f"serving: plain FP32 closed run {cs['span_s']:.1f}s for {n} cases ({cs['cases_per_hour']:.0f} cases/h)")
-    closed_b = run_batched(reqs, storage=N.LB_STORAGE_FP16, setup_workers=setup_workers)
+    closed_b = run_batched(reqs, storage=N.LB_STORAGE_FP16, setup_workers=setup_workers, log=plog)
     cb = summary(closed_b)
     log(f"serving: FP16 batched closed run {cb['span_s']:.1f}s ({cb['cases_per_hour']:.0f} cases/h)")
     gaps = np.random.default_rng(0).exponential(1.0 / (load * mu), size=n)
     gaps[0] = 0.0
     for r, t in zip(reqs, np.cumsum(gaps)):
         r.arrival = float(t)
-    seq = run_sequential(reqs, storage=N.LB_STORAGE_FP32)
-    bat = run_batched(reqs, storage=N.LB_STORAGE_FP16, setup_workers=setup_workers)
+    seq = run_sequential(reqs, storage=N.LB_STORAGE_FP32, log=plog)
+    bat = run_batched(reqs, storage=N.LB_STORAGE_FP16, setup_workers=setup_workers, log=plog)
     ss, bs = summary(seq), summary(bat)
     dcd = [abs(a.cd - b.cd) / max(abs(a.cd), 1e-12) for a, b in zip(seq, bat)]
     log(f"serving: sequential FP32 p50 {ss['p50_s']:.2f}s p99 {ss['p99_s']:.2f}s {ss['cases_per_hour']:.0f} cases/h; "
@@ -366,7 +384,7 @@ def main() -> int:
     ap.add_argument("--workers", type=int, default=4)
     a = ap.parse_args()
     res = compare(a.requests, (a.min_cells, a.max_cells), a.finest_steps, a.load, a.workers,
-                  log=lambda m: print(m, flush=True))
+                  log=lambda m: print(m, flush=True), verbose=True)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -51,8 +51,8 @@ def test_engine_matches_step_batch(hip_device, storage):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("graph,sched", [(True, N.LB_SCHED_XCD), (False, N.LB_SCHED_LINEAR)])
-def test_engine_alone_vs_mixed(hip_device, graph, sched):
+@pytest.mark.parametrize("graph,sched,waves", [(True, N.LB_SCHED_XCD, 5), (False, N.LB_SCHED_LINEAR, 6)])
+def test_engine_alone_vs_mixed(hip_device, graph, sched, waves):
     specs = [((8, 2), 24), ((16, 3), 12), ((8, 3), 16)]
     built = [(_case(*rl), n) for rl, n in specs]
     alone = []
@@ -60,7 +60,7 @@ def test_engine_alone_vs_mixed(hip_device, graph, sched):
         s = Solver(cfg, grids, params, mesh, storage=N.LB_STORAGE_FP16)
         s.init_equilibrium()
         e = Engine(storage=N.LB_STORAGE_FP16)
-        e.set_options(graph=graph, schedule=sched)
+        e.set_options(graph=graph, schedule=sched, waves=waves)
         e.add(s, n)
         e.run_to_completion()
         e.remove(s)
@@ -68,7 +68,7 @@ def test_engine_alone_vs_mixed(hip_device, graph, sched):
         s.close()
         e.close()
     e = Engine(storage=N.LB_STORAGE_FP16)
-    e.set_options(graph=graph, schedule=sched)
+    e.set_options(graph=graph, schedule=sched, waves=waves)
     sol = []
     for (cfg, grids, params, mesh), n in built:
         s = Solver(cfg, grids, params, mesh, storage=N.LB_STORAGE_FP16)
@@ -98,3 +98,25 @@ def test_footprint_is_exact(hip_device, storage):
     e = Engine(storage=storage)
     e.add(s, 16)
     assert footprint(cfg, grids, params, mesh, storage, len(ramp_table(cfg, 16))) == s.device_bytes
+
+
+@pytest.mark.gpu
+def test_autotune_keeps_results(hip_device):
+    """The autotuner's trial steps are real steps under different launch
+    options; the case ends bit-identical to an untuned run."""
+    cfg, grids, params, mesh = _case(16, 3)
+    n = 64
+    ref = Solver(cfg, grids, params, mesh, storage=N.LB_STORAGE_FP16)
+    ref.init_equilibrium()
+    ref.run(n)
+    a = _fields(ref, grids)
+    s = Solver(cfg, grids, params, mesh, storage=N.LB_STORAGE_FP16)
+    s.init_equilibrium()
+    e = Engine(storage=N.LB_STORAGE_FP16)
+    e.add(s, n)
+    res = e.autotune(trial_steps=1)
+    assert len(res["trials"]) >= 3 and res["options"] in [{k: t[k] for k in ("graph", "schedule", "waves")}
+                                                          for t in res["trials"]]
+    e.run_to_completion()
+    e.remove(s)
+    _assert_same(a, _fields(s, grids), "autotuned engine vs step_batch")
