@@ -7,6 +7,9 @@ finest level, 4 levels of 2:1 refinement, 52.5 M active cells in 8^3 blocks,
 inlet u = 0.05 (lattice).  Synthetic: the geometry is generated, the flow
 starts from rest as the reference's init_eq! does.
 
+--workload naca-thermal measures the compressible thermal form instead: an
+extruded NACA 0012 wing at alpha 2 deg with the thermal DDF, 29.4 M cells.
+
 A "step" is one coarse time step: the full multilevel recursion (15
 substeps over 4 levels), issued in batches of 8 like execute_timestep_batch!.
 value = MLUPS_true (every substep's cell updates counted) over the timed
@@ -14,8 +17,11 @@ steps, with the populations resident in HBM.  roofline = the finest level's
 stream-collide kernel (the dominant one): algorithmic bytes per launch
 (253 B per cell update: 2x27x4 B populations + 33 B macroscopic/flags + 4 B
 wall distance; 145 B with --storage fp16/bf16, populations as 16-bit offsets
-f - w) / its average launch time from HIP events on the library's stream.  cpu_baseline = the C oracle (the restated reference algorithm) on
-the host cores over one coarse step of the same workload.
+f - w) / its average launch time from HIP events on the library's stream.
+cpu_baseline = the C oracle (the restated reference algorithm) on the host
+cores over one coarse step of the same workload.  serving = the serving
+comparison of openludwig_amd.serving.compare (FP16 continuous batching vs
+plain FP32 sequential) on a small request set.
 """
 from __future__ import annotations
 
@@ -45,7 +51,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=8)
-    ap.add_argument("--resolution", type=int, default=160)
+    ap.add_argument("--workload", default="cube", choices=["cube", "naca-thermal"],
+                    help="cube: plain drop-in case (52.5M cells); naca-thermal: compressible thermal wing (29.4M cells)")
+    ap.add_argument("--resolution", type=int, default=0, help="0: the workload's own (cube 160, wing 288)")
     ap.add_argument("--strict", action="store_true", help="use the no-FMA build")
     ap.add_argument("--storage", default="fp32", choices=sorted(B_ALG_WMLES),
                     help="population storage (fp16/bf16: offsets f - w, FP32 arithmetic)")
@@ -63,7 +71,15 @@ def main() -> int:
     from openludwig_amd.solver import Solver, build_case, mlups, ramp_velocity
 
     t0 = time.time()
-    cfg, mesh = cases.cube_re1e5(args.resolution)
+    if args.workload == "cube":
+        args.resolution = args.resolution or 160
+        cfg, mesh = cases.cube_re1e5(args.resolution)
+        wname = "cube Re 1e5, 49,152-tri cube, cumulant+WALE+WMLES, Bouzidi finest level"
+    else:
+        args.resolution = args.resolution or 288
+        cfg, mesh = cases.naca_wing_thermal(args.resolution)
+        wname = ("NACA 0012 wing (chord 1, span 2, 40,400 tris, alpha 2), cumulant+WALE+WMLES, thermal DDF + "
+                 "O(u^4) equilibrium, u_lat 0.15, Bouzidi finest level")
     grids, params, mesh = build_case(cfg, mesh)
     n_cells = [g.n_cells for g in grids]
     log(f"setup {time.time() - t0:.1f}s: levels {len(grids)}, cells {[f'{c / 1e6:.2f}M' for c in n_cells]}, "
@@ -91,12 +107,14 @@ def main() -> int:
     solver.profile(True)
     for g in grids:
         solver.profile_read(g.level_id)
+        solver.profile_read(g.level_id, 1)
     sync()
     t_start = time.perf_counter()
     run(1 + args.warmup, args.steps)
     sync()
     elapsed = time.perf_counter() - t_start
     kern = {g.level_id: solver.profile_read(g.level_id) for g in grids}
+    kern_t = {g.level_id: solver.profile_read(g.level_id, 1) for g in grids}
     solver.profile(False)
     st = solver.stats(1)
     if st["nonfinite"]:
@@ -109,12 +127,19 @@ def main() -> int:
     traffic = None
     prof = ROOT / "profiles" / ("collide_cube_re1e5_52M.json" if args.storage == "fp32"
                                 else f"collide_cube_re1e5_52M_{args.storage}.json")
-    if prof.exists() and not args.strict and args.resolution == 160:
+    if prof.exists() and not args.strict and args.workload == "cube" and args.resolution == 160:
         pj = json.loads(prof.read_text())
         if pj.get("cells") == grids[-1].n_cells:
             traffic = round(pj["read_bytes_per_launch"] + pj["write_bytes_per_launch"])
 
     m_true, m_ref = mlups(grids, args.steps, elapsed)
+    thermal = None
+    if cfg.ddf_enabled:
+        # thermal DDF kernel: 2x27x4 B populations + u 12 + T 4 + obstacle 1 + sponge 4
+        tms, tn = kern_t[finest]
+        t_avg = tms / max(tn, 1)
+        thermal = {"bytes_per_lu": 237, "finest_avg_launch_ms": round(t_avg, 4),
+                   "finest_GBps": round(grids[-1].n_cells * 237 / (t_avg * 1e-3) / 1e9, 1)}
     ms_k, n_k = kern[finest]
     avg_ms = ms_k / max(n_k, 1)
     bytes_launch = grids[-1].n_cells * b_alg
@@ -139,7 +164,8 @@ def main() -> int:
         o.close()
         c_true, _ = mlups(grids, args.cpu_steps, tcpu)
         cpu = {"value": round(c_true, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} coarse step(s) of the same 52.5M-cell, 4-level workload "
+               "sample": f"{args.cpu_steps} coarse step(s) of the same {sum(n_cells) / 1e6:.1f}M-cell, "
+                         f"{len(grids)}-level workload "
                          f"({sum(c * 2 ** i for i, c in enumerate(n_cells)) * args.cpu_steps / 1e6:.0f}M cell "
                          f"updates), C oracle (FP32 mirror, OpenMP)", "seconds": round(tcpu, 2)}
         log(f"cpu baseline {c_true:.1f} MLUPS on {threads} threads ({tcpu:.1f}s)")
@@ -164,14 +190,13 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "cube Re 1e5, 49,152-tri cube, cumulant+WALE+WMLES, Bouzidi finest level, "
-                               f"{len(grids)} levels, {sum(n_cells) / 1e6:.1f}M cells, u_lat 0.05, "
+        "config": {"workload": f"{wname}, {len(grids)} levels, {sum(n_cells) / 1e6:.1f}M cells, "
                                f"resolution {args.resolution}",
                    "cells_per_level": n_cells, "lu_per_step": sum(c * 2 ** i for i, c in enumerate(n_cells)),
                    "mlups_ref": round(m_ref, 1), "build": "strict" if args.strict else "fast",
                    "population_storage": args.storage, "bytes_per_lu": b_alg,
                    "hbm_roofline_pct_step": round(m_true * 1e6 * b_alg / (HBM_PEAK_GBS * 1e9) * 100, 2),
-                   "per_level_collide": per_level, "Cd": round(forces["Cd"], 5)},
+                   "per_level_collide": per_level, "Cd": round(forces["Cd"], 5), "thermal": thermal},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "collide_kernel (finest level)", "bytes_per_launch": bytes_launch,

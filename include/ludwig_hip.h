@@ -210,6 +210,8 @@ int lb_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
  * last read, and resets them.                                              */
 int lb_profile_enable(lb_ctx* ctx, int enable);
 int lb_profile_read(lb_ctx* ctx, int level_id, double* ms, int64_t* launches);
+/* The same for kernel kind `kernel`: 0 stream-collide, 1 thermal DDF.     */
+int lb_profile_read_kernel(lb_ctx* ctx, int level_id, int kernel, double* ms, int64_t* launches);
 
 /* ------------------------------------------------------------------------
  * Batched engine (no counterpart in the reference, whose run_all_cases,

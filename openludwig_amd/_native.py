@@ -82,6 +82,7 @@ HIP_EXPORTS = {
     "lb_device_bytes": (C.c_int64, [C.c_void_p]),
     "lb_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "lb_profile_read": (C.c_int, [C.c_void_p, C.c_int, f64p, i64p]),
+    "lb_profile_read_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int, f64p, i64p]),
     "lb_bandwidth_probe": (C.c_int, [C.c_int, C.c_int64, C.c_int, f64p]),
     "lb_engine_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int32, C.c_int32]),
     "lb_engine_destroy": (None, [C.c_void_p]),

@@ -104,8 +104,9 @@ struct Level {
     std::vector<int32_t> order_host;  // processing order (engine work lists)
     int32_t n_int = 0;                // interior blocks: the first n_int of the order
     // kernel timing: event pairs recorded around collide launches
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    size_t ev_used = 0;
+    // [0] stream-collide launches, [1] thermal launches
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[2];
+    size_t ev_used[2] = {0, 0};
 };
 
 }  // namespace
@@ -395,6 +396,25 @@ int build_links(lb_ctx* c, Level& L, const lb_level_desc* d) {
 // solver_control.jl:6-197): substep j of level li in the coarse step held by
 // the case record (t_sub = t0 * 2^li + j), then both child substeps.  The
 // kernels resolve buffers, parity, seed and temporal weight from (record, j).
+// Kernel timing: record the start event of kernel kind `which` (0 collide,
+// 1 thermal) and hand back the end event to record after the launch.
+int profile_begin(lb_ctx* c, Level& L, int which, hipEvent_t* end) {
+    *end = nullptr;
+    if (!c->profile) return LB_OK;
+    auto& ev = L.ev[which];
+    size_t& used = L.ev_used[which];
+    if (used == ev.size()) {
+        hipEvent_t x, y;
+        HIPCHK(c, hipEventCreate(&x));
+        HIPCHK(c, hipEventCreate(&y));
+        ev.emplace_back(x, y);
+    }
+    HIPCHK(c, hipEventRecord(ev[used].first, c->stream));
+    *end = ev[used].second;
+    used++;
+    return LB_OK;
+}
+
 int step_level(lb_ctx* c, int li, int64_t j) {
     if (li >= (int)c->levels.size()) return LB_OK;
     Level& L = c->levels[(size_t)li];
@@ -405,17 +425,7 @@ int step_level(lb_ctx* c, int li, int64_t j) {
     if (L.nb > 0) {
         if (li > 0 && L.n_gchunks > 0) HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 0, c->stream));
         hipEvent_t e1 = nullptr;
-        if (c->profile) {
-            if (L.ev_used == L.ev.size()) {
-                hipEvent_t x, y;
-                HIPCHK(c, hipEventCreate(&x));
-                HIPCHK(c, hipEventCreate(&y));
-                L.ev.emplace_back(x, y);
-            }
-            HIPCHK(c, hipEventRecord(L.ev[L.ev_used].first, c->stream));
-            e1 = L.ev[L.ev_used].second;
-            L.ev_used++;
-        }
+        RC(profile_begin(c, L, 0, &e1));
         HIPCHK(c, c->k->collide(las, st, L.wcol, L.n_int, j, storage, L.wide ? 1 : 0, 0, 0, 5, c->stream));
         HIPCHK(c, c->k->collide(las, st, L.wcol + L.n_int, L.nb - L.n_int, j, storage, L.wide ? 1 : 0, 1, 0, 5,
                                 c->stream));
@@ -424,7 +434,10 @@ int step_level(lb_ctx* c, int li, int64_t j) {
         if (ddf) {
             if (li > 0 && L.n_gchunks > 0)
                 HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 1, c->stream));
+            hipEvent_t e2 = nullptr;
+            RC(profile_begin(c, L, 1, &e2));
             HIPCHK(c, c->k->thermal(las, st, L.wcol, L.nb, j, L.wide_g ? 1 : 0, 0, c->stream));
+            if (e2) HIPCHK(c, hipEventRecord(e2, c->stream));
             if (L.n_lchunks > 0) HIPCHK(c, c->k->bouzidi(las, st, L.wlk, L.n_lchunks, j, storage, 1, c->stream));
         }
     }
@@ -746,10 +759,11 @@ void lb_destroy(lb_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& L : c->levels)
-        for (auto& e : L.ev) {
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
-        }
+        for (auto& evk : L.ev)
+            for (auto& e : evk) {
+                (void)hipEventDestroy(e.first);
+                (void)hipEventDestroy(e.second);
+            }
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1185,21 +1199,26 @@ int lb_profile_enable(lb_ctx* c, int enable) {
     return LB_OK;
 }
 
-int lb_profile_read(lb_ctx* c, int level_id, double* ms, int64_t* launches) {
-    if (!c || !ms || !launches || level_id < 1 || level_id > (int)c->levels.size()) return LB_EINVAL;
+int lb_profile_read_kernel(lb_ctx* c, int level_id, int kernel, double* ms, int64_t* launches) {
+    if (!c || !ms || !launches || level_id < 1 || level_id > (int)c->levels.size() || kernel < 0 || kernel > 1)
+        return LB_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     Level& L = c->levels[(size_t)level_id - 1];
     double tot = 0.0;
-    for (size_t i = 0; i < L.ev_used; i++) {
+    for (size_t i = 0; i < L.ev_used[kernel]; i++) {
         float t = 0.0f;
-        HIPCHK(c, hipEventElapsedTime(&t, L.ev[i].first, L.ev[i].second));
+        HIPCHK(c, hipEventElapsedTime(&t, L.ev[kernel][i].first, L.ev[kernel][i].second));
         tot += (double)t;
     }
     *ms = tot;
-    *launches = (int64_t)L.ev_used;
-    L.ev_used = 0;
+    *launches = (int64_t)L.ev_used[kernel];
+    L.ev_used[kernel] = 0;
     return LB_OK;
+}
+
+int lb_profile_read(lb_ctx* c, int level_id, double* ms, int64_t* launches) {
+    return lb_profile_read_kernel(c, level_id, 0, ms, launches);
 }
 
 int lb_engine_create(lb_engine** out, int device, int32_t storage, int32_t strict_fp) {

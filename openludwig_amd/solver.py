@@ -242,11 +242,13 @@ class Solver:
     def profile(self, enable: bool = True) -> None:
         self._check(self.lib.lb_profile_enable(self._ctx, int(enable)), "lb_profile_enable")
 
-    def profile_read(self, level_id: int) -> tuple:
-        """(summed collide kernel ms, launches) of a level since the last read."""
+    def profile_read(self, level_id: int, kernel: int = 0) -> tuple:
+        """(summed kernel ms, launches) of a level since the last read;
+        kernel 0 = stream-collide, 1 = thermal DDF."""
         ms = C.c_double()
         n = C.c_int64()
-        self._check(self.lib.lb_profile_read(self._ctx, level_id, C.byref(ms), C.byref(n)), "lb_profile_read")
+        self._check(self.lib.lb_profile_read_kernel(self._ctx, level_id, kernel, C.byref(ms), C.byref(n)),
+                    "lb_profile_read_kernel")
         return ms.value, n.value
 
     def stats(self, level_id: int = 1) -> dict:
