@@ -135,11 +135,12 @@ def main() -> int:
     m_true, m_ref = mlups(grids, args.steps, elapsed)
     thermal = None
     if cfg.ddf_enabled:
-        # thermal DDF kernel: 2x27x4 B populations + u 12 + T 4 + obstacle 1 + sponge 4
+        # thermal DDF kernel: 2x27 populations (4 or 2 B) + u 12 + T 4 + obstacle 1 + sponge 4
         tms, tn = kern_t[finest]
         t_avg = tms / max(tn, 1)
-        thermal = {"bytes_per_lu": 237, "finest_avg_launch_ms": round(t_avg, 4),
-                   "finest_GBps": round(grids[-1].n_cells * 237 / (t_avg * 1e-3) / 1e9, 1)}
+        tb = 2 * 27 * (4 if args.storage == "fp32" else 2) + 21
+        thermal = {"bytes_per_lu": tb, "finest_avg_launch_ms": round(t_avg, 4),
+                   "finest_GBps": round(grids[-1].n_cells * tb / (t_avg * 1e-3) / 1e9, 1)}
     ms_k, n_k = kern[finest]
     avg_ms = ms_k / max(n_k, 1)
     bytes_launch = grids[-1].n_cells * b_alg

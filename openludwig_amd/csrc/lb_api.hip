@@ -40,7 +40,7 @@ struct Kern {
     hipError_t (*collide)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
                           int, int, int, hipStream_t);
     hipError_t (*thermal)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
-                          hipStream_t);
+                          int, hipStream_t);
     hipError_t (*ghost_fill)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
                              hipStream_t);
     hipError_t (*bouzidi)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
@@ -48,7 +48,7 @@ struct Kern {
     hipError_t (*set_state)(lbk::CaseState*, const lbk::CaseState&, hipStream_t);
     hipError_t (*advance)(lbk::CaseState*, int32_t, hipStream_t);
     hipError_t (*init_level)(void*, void*, float*, float*, float*, float*, int32_t, int, hipStream_t);
-    hipError_t (*init_thermal)(float*, float*, float*, float*, int32_t, float, hipStream_t);
+    hipError_t (*init_thermal)(void*, void*, float*, float*, int32_t, float, int, hipStream_t);
     hipError_t (*forces)(const lbk::ForceArgs&, int, double*, hipStream_t);
     hipError_t (*stats)(const void*, const float*, const float*, const uint8_t*, int32_t, int, double*, int,
                         hipStream_t);
@@ -69,7 +69,7 @@ struct Level {
     void* f[2] = {nullptr, nullptr};  // 4 or 2 bytes per population (lb_config.storage)
     float* rho[2] = {nullptr, nullptr};
     float* vel[2] = {nullptr, nullptr};
-    float* g[2] = {nullptr, nullptr};
+    void* g[2] = {nullptr, nullptr};  // thermal populations, same storage as f
     float* T[2] = {nullptr, nullptr};
     uint8_t* obst = nullptr;
     float* sponge = nullptr;
@@ -436,7 +436,7 @@ int step_level(lb_ctx* c, int li, int64_t j) {
                 HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 1, c->stream));
             hipEvent_t e2 = nullptr;
             RC(profile_begin(c, L, 1, &e2));
-            HIPCHK(c, c->k->thermal(las, st, L.wcol, L.nb, j, L.wide_g ? 1 : 0, 0, c->stream));
+            HIPCHK(c, c->k->thermal(las, st, L.wcol, L.nb, j, storage, L.wide_g ? 1 : 0, 0, c->stream));
             if (e2) HIPCHK(c, hipEventRecord(e2, c->stream));
             if (L.n_lchunks > 0) HIPCHK(c, c->k->bouzidi(las, st, L.wlk, L.n_lchunks, j, storage, 1, c->stream));
         }
@@ -689,7 +689,7 @@ hipError_t engine_op(lb_engine* e, int d, int64_t j) {
     if (W.nlk) ETRY(k->bouzidi(e->las, st, W.lk, W.nlk, j, e->storage, 0, e->stream));
     if (W.ntcol) {
         if (W.ntgh) ETRY(k->ghost_fill(e->las, st, W.tgh, W.ntgh, j, e->storage, 1, e->stream));
-        ETRY(k->thermal(e->las, st, W.tcol, W.ntcol, j, W.wide_g, e->sched, e->stream));
+        ETRY(k->thermal(e->las, st, W.tcol, W.ntcol, j, e->storage, W.wide_g, e->sched, e->stream));
         if (W.ntlk) ETRY(k->bouzidi(e->las, st, W.tlk, W.ntlk, j, e->storage, 1, e->stream));
     }
     if (d < e->lmax) {
@@ -850,7 +850,7 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     L.ngent = (int64_t)gent.size() / 2;
     const size_t n4 = (size_t)nb * NC, n5 = ((size_t)nb + ngh) * NC * Q;
     L.wide = (n5 + 512) * pop_bytes(c->cfg.storage) >= ((size_t)1 << 32);
-    L.wide_g = (n5 + 512) * 4 >= ((size_t)1 << 32);
+    L.wide_g = L.wide;  // thermal populations use the same storage
     for (int i = 0; i < 2; i++) {
         // 512 slots of padding: pulls from a missing neighbour block read a
         // harmless slot up to 200 past the cell's own block
@@ -860,7 +860,9 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
         RC(dalloc(c, &L.rho[i], n4));
         RC(dalloc(c, &L.vel[i], n4 * 3));
         if (c->cfg.ddf_enabled) {
-            RC(dalloc(c, &L.g[i], n5 + 512));
+            uint8_t* gb = nullptr;
+            RC(dalloc(c, &gb, (n5 + 512) * pop_bytes(c->cfg.storage)));
+            L.g[i] = gb;
             RC(dalloc(c, &L.T[i], n4));
         }
     }
@@ -942,7 +944,8 @@ int lb_init_equilibrium(lb_ctx* c, float T0) {
     for (auto& L : c->levels) {
         HIPCHK(c, c->k->init_level(L.f[0], L.f[1], L.rho[0], L.rho[1], L.vel[0], L.vel[1], L.nb, c->cfg.storage,
                                    c->stream));
-        if (c->cfg.ddf_enabled) HIPCHK(c, c->k->init_thermal(L.g[0], L.g[1], L.T[0], L.T[1], L.nb, T0, c->stream));
+        if (c->cfg.ddf_enabled)
+            HIPCHK(c, c->k->init_thermal(L.g[0], L.g[1], L.T[0], L.T[1], L.nb, T0, c->cfg.storage, c->stream));
         L.cur = 0;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1074,9 +1077,9 @@ int lb_download_level(lb_ctx* c, int level_id, int field, float* dst) {
     switch (field) {
         case LB_FIELD_F:
         case LB_FIELD_G: {
-            const void* src = field == LB_FIELD_F ? L.f[L.cur] : (const void*)L.g[L.cur];
+            const void* src = field == LB_FIELD_F ? L.f[L.cur] : L.g[L.cur];
             if (!src) return fail(c, LB_EINVAL, "field not allocated");
-            const int st = field == LB_FIELD_F ? c->cfg.storage : LB_STORAGE_FP32;
+            const int st = c->cfg.storage;
             std::vector<uint8_t> tmp(n4 * Q * pop_bytes(st));
             HIPCHK(c, hipMemcpy(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost));
             for (int k = 0; k < Q; k++) {
@@ -1122,10 +1125,9 @@ int lb_upload_level(lb_ctx* c, int level_id, int field, const float* src) {
     switch (field) {
         case LB_FIELD_F:
         case LB_FIELD_G: {
-            void* const dev[2] = {field == LB_FIELD_F ? L.f[0] : (void*)L.g[0],
-                                  field == LB_FIELD_F ? L.f[1] : (void*)L.g[1]};
+            void* const dev[2] = {field == LB_FIELD_F ? L.f[0] : L.g[0], field == LB_FIELD_F ? L.f[1] : L.g[1]};
             if (!dev[0]) return fail(c, LB_EINVAL, "field not allocated");
-            const int st = field == LB_FIELD_F ? c->cfg.storage : LB_STORAGE_FP32;
+            const int st = c->cfg.storage;
             std::vector<uint8_t> raw(n4 * Q * pop_bytes(st));
             for (int k = 0; k < Q; k++) {
                 const float w = wk_host(k);

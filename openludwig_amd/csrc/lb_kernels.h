@@ -62,7 +62,7 @@ struct LevelArgs {
     void* f[2];  // ping-pong pairs, indexed by substep parity (input = t_sub & 1)
     float* rho[2];
     float* vel[2];
-    float* g[2];  // thermal populations (FP32) or null
+    void* g[2];  // thermal populations (storage as f) or null
     float* T[2];
     const uint8_t* obst;
     const float* sponge;
@@ -77,7 +77,7 @@ struct LevelArgs {
     const void* pf[2];
     const float* prho[2];
     const float* pvel[2];
-    const float* pg[2];
+    const void* pg[2];
     const float* pT[2];
     const int32_t* pgrid;
     int32_t pdx, pdy, pdz;
@@ -136,8 +136,8 @@ struct ForceArgs {
     hipError_t collide(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work,    \
                        int32_t n_tiles, int64_t j, int storage, int wide, int boundary, int sched,       \
                        int waves, hipStream_t s);                                                        \
-    hipError_t thermal(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work, int32_t n_tiles, int64_t j,      \
-                       int wide, int sched, hipStream_t s);                                              \
+    hipError_t thermal(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work,    \
+                       int32_t n_tiles, int64_t j, int storage, int wide, int sched, hipStream_t s);    \
     hipError_t ghost_fill(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work, int32_t n_chunks, int64_t j,  \
                           int storage, int thermal, hipStream_t s);                                      \
     hipError_t bouzidi(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work, int32_t n_chunks, int64_t j,     \
@@ -146,7 +146,7 @@ struct ForceArgs {
     hipError_t advance(lbk::CaseState* st, int32_t n, hipStream_t s);                             \
     hipError_t init_level(void* f0, void* f1, float* rho0, float* rho1, float* v0, float* v1, int32_t nb,  \
                           int storage, hipStream_t s);                                                   \
-    hipError_t init_thermal(float* g0, float* g1, float* T0, float* T1, int32_t nb, float Tinit,         \
+    hipError_t init_thermal(void* g0, void* g1, float* T0, float* T1, int32_t nb, float Tinit, int storage, \
                             hipStream_t s);                                                              \
     hipError_t forces(const lbk::ForceArgs& a, int n_groups, double* out9, hipStream_t s);               \
     hipError_t stats(const void* f, const float* rho, const float* vel, const uint8_t* obst, int32_t nb,  \

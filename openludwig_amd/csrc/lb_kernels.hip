@@ -73,8 +73,8 @@ struct CollideArgs {
 };
 
 struct ThermalArgs {
-    const float* g_in;
-    float* g_out;
+    const void* g_in;  // thermal populations, storage as the flow's (offsets from w_k in 16 bits)
+    void* g_out;
     const float* vel;  // the flow step's vel_out
     float* T_out;
     const uint8_t* obst;
@@ -918,17 +918,18 @@ hipError_t collide(const LevelArgs* las, const CaseState* states, const Work* wo
 
 // ---------------------------------------------------------------------------
 // Thermal DDF (src/collision/kernel_thermal.jl:14-202)
+template <int ST>
 __device__ __forceinline__ float thermal_bc(const ThermalArgs& a, int k, int b, int c, int sgx, int sgy, int sgz) {
     float cx = (float)CX(k), cy = (float)CY(k), cz = (float)CZ(k), w = wk_rt(k);
     if (sgx < 1) return geq(a.T_inlet, a.u_inlet, 0.0f, 0.0f, w, cx, cy, cz);
     if (sgx > a.nxg) return geq(a.T_inlet, a.u_inlet, 0.0f, 0.0f, w, cx, cy, cz);
-    if (sgy < 1 || sgy > a.nyg) return a.g_in[fidx(b, MIRY(k), c)];
-    if (sgz < 1 || sgz > a.nzg) return a.g_in[fidx(b, MIRZ(k), c)];
+    if (sgy < 1 || sgy > a.nyg) return Codec<ST>::load(a.g_in, fidx(b, MIRY(k), c), w);
+    if (sgz < 1 || sgz > a.nzg) return Codec<ST>::load(a.g_in, fidx(b, MIRZ(k), c), w);
     // in-domain sources without a block are ghost blocks (thermal ghost fill)
     return w * a.T_inlet;
 }
 
-template <class OffT>
+template <int ST, class OffT>
 __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restrict__ las,
                                                       const CaseState* __restrict__ states,
                                                       const Work* __restrict__ work, int32_t n_tiles, int64_t j,
@@ -942,7 +943,8 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
     const LevelArgs& L = las[wk.li];
     const int b = wk.item;
     const int c = (half << 8) | threadIdx.x;
-    build_pull_table<OffT>(wk.nbr_row, b, half, (OffT)4, ptab, smask);
+    constexpr int ESZ = ST == 0 ? 4 : 2;
+    build_pull_table<OffT>(wk.nbr_row, b, half, (OffT)ESZ, ptab, smask);
     const CaseState S = case_state(states + wk.ci);
     __syncthreads();
     if (!live(L, S, j)) return;
@@ -958,7 +960,7 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
     float ux = gp(a.vel)[iv], uy = gp(a.vel)[iv + NC], uz = gp(a.vel)[iv + 2 * NC];
     const int oxp = x == 0 ? -1 : 0, oxm = x == 7 ? 1 : 0;
     const int oyp = y == 0 ? -1 : 0, oym = y == 7 ? 1 : 0;
-    const OffT cb = (OffT)c * (OffT)4;
+    const OffT cb = (OffT)c * (OffT)ESZ;
     float G[27];
 #pragma unroll
     for (int k = 0; k < 27; k++) {
@@ -966,8 +968,8 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
         const int ox = cx == 1 ? oxp : (cx == -1 ? oxm : 0);
         const int oy = cy == 1 ? oyp : (cy == -1 ? oym : 0);
         const OffT off = ptab[wv * 27 + 9 * (cz + 1) + (ox + 1) + 3 * (oy + 1)] + cb +
-                         (OffT)((512 * k - (cx + 8 * cy + 64 * cz)) * 4);
-        G[k] = *gaddr(a.g_in, off);
+                         (OffT)((512 * k - (cx + 8 * cy + 64 * cz)) * ESZ);
+        G[k] = Codec<ST>::loadb(a.g_in, off, WK(k));
     }
     uint32_t miss = 0u;
     const uint32_t m = wave_miss(smask, wv);
@@ -983,7 +985,8 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
     if (miss != 0u) {
 #pragma unroll 1
         for (int k = 0; k < 27; k++)
-            if (miss & (1u << k)) sfix[k * 256 + threadIdx.x] = thermal_bc(a, k, b, c, gx - CX(k), gy - CY(k), gz - CZ(k));
+            if (miss & (1u << k))
+                sfix[k * 256 + threadIdx.x] = thermal_bc<ST>(a, k, b, c, gx - CX(k), gy - CY(k), gz - CZ(k));
 #pragma unroll
         for (int k = 0; k < 27; k++)
             if (miss & (1u << k)) G[k] = sfix[k * 256 + threadIdx.x];
@@ -994,7 +997,7 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
     if (is_obs) {
 #pragma unroll
         for (int k = 0; k < 27; k++) {
-            float gopp = a.g_in[fidx(b, OPP(k), c)];  // local, unstreamed (kernel_thermal.jl:140)
+            float gopp = Codec<ST>::load(a.g_in, fidx(b, OPP(k), c), WK(k));  // local, unstreamed (kernel_thermal.jl:140)
             float go;
             if (a.adiabatic == 1) {
                 go = gopp;
@@ -1002,7 +1005,7 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
                 float gw = geq(a.T_wall, 0.0f, 0.0f, 0.0f, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k));
                 go = 2.0f * gw - gopp;
             }
-            a.g_out[fidx(b, k, c)] = go;
+            Codec<ST>::store(a.g_out, fidx(b, k, c), go, WK(k));
         }
         a.T_out[i4] = a.adiabatic == 1 ? Tsum : a.T_wall;
         return;
@@ -1025,16 +1028,23 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
             gs = gs * (1.0f - sp) + gt * sp;
         }
         float ge = geq(Tl, ux, uy, uz, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k));
-        a.g_out[fidx(b, k, c)] = gs - omg * (gs - ge);
+        Codec<ST>::store(a.g_out, fidx(b, k, c), gs - omg * (gs - ge), WK(k));
     }
 }
 
-hipError_t thermal(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_tiles, int64_t j, int wide, int sched,
-                   hipStream_t s) {
-    if (n_tiles <= 0) return hipSuccess;
+template <int ST>
+void thermal_launch(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_tiles, int64_t j,
+                    int wide, int sched, hipStream_t s) {
     const dim3 g(padded_groups(n_tiles)), t(256);
-    if (wide) hipLaunchKernelGGL(thermal_kernel<uint64_t>, g, t, 0, s, las, states, work, n_tiles, j, sched);
-    else hipLaunchKernelGGL(thermal_kernel<uint32_t>, g, t, 0, s, las, states, work, n_tiles, j, sched);
+    if (wide) hipLaunchKernelGGL((thermal_kernel<ST, uint64_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+    else hipLaunchKernelGGL((thermal_kernel<ST, uint32_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+}
+hipError_t thermal(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_tiles, int64_t j,
+                   int storage, int wide, int sched, hipStream_t s) {
+    if (n_tiles <= 0) return hipSuccess;
+    if (storage == 1) thermal_launch<1>(las, states, work, n_tiles, j, wide, sched, s);
+    else if (storage == 2) thermal_launch<2>(las, states, work, n_tiles, j, wide, sched, s);
+    else thermal_launch<0>(las, states, work, n_tiles, j, wide, sched, s);
     return hipGetLastError();
 }
 
@@ -1166,12 +1176,12 @@ __global__ void __launch_bounds__(256) ghost_kernel(const LevelArgs* __restrict_
 }
 
 // One work-group per work entry (LevelArgs index, first of 256 ghost entries).
-// Thermal populations are FP32 whatever the flow storage.
-hipError_t ghost_fill(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_chunks, int64_t j, int storage, int thermal,
-                      hipStream_t s) {
+// Thermal populations use the flow's storage.
+hipError_t ghost_fill(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_chunks, int64_t j,
+                      int storage, int thermal, hipStream_t s) {
     if (n_chunks <= 0) return hipSuccess;
     const dim3 grid((unsigned)n_chunks), t(256);
-    const int st = thermal ? 0 : storage;
+    const int st = storage;
     if (st == 1) hipLaunchKernelGGL(ghost_kernel<1>, grid, t, 0, s, las, states, work, j, thermal);
     else if (st == 2) hipLaunchKernelGGL(ghost_kernel<2>, grid, t, 0, s, las, states, work, j, thermal);
     else hipLaunchKernelGGL(ghost_kernel<0>, grid, t, 0, s, las, states, work, j, thermal);
@@ -1236,7 +1246,7 @@ void bouzidi_launch(const LevelArgs* las, const CaseState* states, const Work* w
 hipError_t bouzidi(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_chunks, int64_t j, int storage, int thermal,
                    hipStream_t s) {
     if (n_chunks <= 0) return hipSuccess;
-    const int st = thermal ? 0 : storage;
+    const int st = storage;
     if (st == 1) bouzidi_launch<1>(las, states, work, n_chunks, j, thermal, s);
     else if (st == 2) bouzidi_launch<2>(las, states, work, n_chunks, j, thermal, s);
     else bouzidi_launch<0>(las, states, work, n_chunks, j, thermal, s);
@@ -1298,21 +1308,26 @@ hipError_t init_level(void* f0, void* f1, float* rho0, float* rho1, float* v0, f
 }
 
 // init_thermal_eq! (main.jl:131-150)
-__global__ void init_thermal_kernel(float* g0, float* g1, float* T0, float* T1, float Tinit) {
+template <int ST>
+__global__ void init_thermal_kernel(void* g0, void* g1, float* T0, float* T1, float Tinit) {
     const int b = blockIdx.x >> 1;
     const int c = ((blockIdx.x & 1) << 8) | threadIdx.x;
 #pragma unroll
     for (int k = 0; k < 27; k++) {
         float ge = WK(k) * Tinit;
-        g0[fidx(b, k, c)] = ge;
-        g1[fidx(b, k, c)] = ge;
+        Codec<ST>::store(g0, fidx(b, k, c), ge, WK(k));
+        Codec<ST>::store(g1, fidx(b, k, c), ge, WK(k));
     }
     T0[(size_t)b * NC + c] = Tinit;
     T1[(size_t)b * NC + c] = Tinit;
 }
-hipError_t init_thermal(float* g0, float* g1, float* T0, float* T1, int32_t nb, float Tinit, hipStream_t s) {
+hipError_t init_thermal(void* g0, void* g1, float* T0, float* T1, int32_t nb, float Tinit, int storage,
+                        hipStream_t s) {
     if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(init_thermal_kernel, dim3((unsigned)nb * 2), dim3(256), 0, s, g0, g1, T0, T1, Tinit);
+    const dim3 grid((unsigned)nb * 2), t(256);
+    if (storage == 1) hipLaunchKernelGGL(init_thermal_kernel<1>, grid, t, 0, s, g0, g1, T0, T1, Tinit);
+    else if (storage == 2) hipLaunchKernelGGL(init_thermal_kernel<2>, grid, t, 0, s, g0, g1, T0, T1, Tinit);
+    else hipLaunchKernelGGL(init_thermal_kernel<0>, grid, t, 0, s, g0, g1, T0, T1, Tinit);
     return hipGetLastError();
 }
 

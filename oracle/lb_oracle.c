@@ -376,11 +376,11 @@ int orc_init_equilibrium(orc_ctx* c, float T0f) {
         if (L_->has_thermal) {
             for (int k = 0; k < 27; k++)
                 for (size_t i = 0; i < n4; i++) {
-                    real ge = W[k] * T0;
+                    real ge = store_round(L_->storage, W[k] * T0, k);
                     L_->g[i + n4 * k] = ge;
                     L_->g_temp[i + n4 * k] = ge;
-                    if (L_->g_old) L_->g_old[i + n4 * k] = W[k] * T0;
-                    if (L_->g_post) L_->g_post[i + n4 * k] = W[k] * T0;
+                    if (L_->g_old) L_->g_old[i + n4 * k] = ge;
+                    if (L_->g_post) L_->g_post[i + n4 * k] = ge;
                 }
             for (size_t i = 0; i < n4; i++) { L_->T[i] = T0; L_->T_old[i] = T0; }
         }
@@ -919,7 +919,9 @@ static void thermal_cell(const orc_level* L_, const orc_parent* pp, const orc_pa
                 else if (sgx > nxg) val = geq(Tin, u_inlet, 0, 0, W[k], (real)cx, (real)cy, (real)cz);
                 else if (sgy < 1 || sgy > nyg) val = g_in[i4 + n4 * MY[k]];
                 else if (sgz < 1 || sgz > nzg) val = g_in[i4 + n4 * MZ[k]];
-                else if (!pp->is_l1) val = interp_rescale(pp, 1, sgx, sgy, sgz, k, L_->tau_g, tw, prm->temporal, 0);
+                else if (!pp->is_l1)
+                    val = store_round(L_->storage,
+                                      interp_rescale(pp, 1, sgx, sgy, sgz, k, L_->tau_g, tw, prm->temporal, 0), k);
                 else val = W[k] * Tin;
             }
         }
@@ -931,6 +933,7 @@ static void thermal_cell(const orc_level* L_, const orc_parent* pp, const orc_pa
                 real gw = geq(Tw, 0, 0, 0, W[k], (real)cx, (real)cy, (real)cz);
                 go = L(2.0) * gw - g_in[i4 + n4 * OPP[k]];
             }
+            go = store_round(L_->storage, go, k);
             g_out[i4 + n4 * k] = go;
             if (store_post) L_->g_post[i4 + n4 * k] = go;
         } else {
@@ -958,7 +961,7 @@ static void thermal_cell(const orc_level* L_, const orc_parent* pp, const orc_pa
             gst = gst * (L(1.0) - sp) + gt * sp;
         }
         real ge = geq(Tl, ux, uy, uz, W[k], (real)CX[k], (real)CY[k], (real)CZ[k]);
-        real gc = gst - omg * (gst - ge);
+        real gc = store_round(L_->storage, gst - omg * (gst - ge), k);
         if (store_post) L_->g_post[i4 + n4 * k] = gc;
         g_out[i4 + n4 * k] = gc;
     }
@@ -989,7 +992,7 @@ static void timestep_level(orc_ctx* c, orc_level* L_, const orc_parent* pp, cons
                     for (int x = 1; x <= BS; x++)
                         thermal_cell(L_, pp, prm, g_out, g_in, vel_out, store_post, u_curr, nxg, nyg, nzg, tw, x, y,
                                      z, b);
-        if (L_->nbc > 0) bouzidi_apply(L_, g_out, L_->g_post, (real)prm->q_min, 0);
+        if (L_->nbc > 0) bouzidi_apply(L_, g_out, L_->g_post, (real)prm->q_min, L_->storage);
     }
 }
 
@@ -1120,7 +1123,7 @@ int orc_put(orc_ctx* c, int lvl, int which, int t_last, const double* in) {
         default: return -3;
     }
     for (size_t i = 0; i < n; i++)
-        dst[i] = which == 0 ? store_round(L_->storage, (real)in[i], (int)(i / n4)) : (real)in[i];
+        dst[i] = (which == 0 || which == 3) ? store_round(L_->storage, (real)in[i], (int)(i / n4)) : (real)in[i];
     return 0;
 }
 

@@ -134,14 +134,16 @@ def test_fast_one_step_and_n_steps(hip_device):
 
 
 @pytest.mark.gpu
-def test_strict_thermal_ddf_bitexact(hip_device):
+@pytest.mark.parametrize("storage", [0, 1, 2])
+def test_strict_thermal_ddf_bitexact(hip_device, storage):
     """Thermal DDF (kernel_thermal.jl) + O(u^4) equilibrium, 2 levels, Bouzidi
-    on g, strict build: g, T and the flow fields bit-identical to the oracle."""
+    on g, strict build: g, T and the flow fields bit-identical to the oracle,
+    with FP32 populations and with FP16 / BF16 offset storage of f and g."""
     cfg, mesh = cases.tiny_sphere(8, 2, ddf=True)
     cfg = cfg.replace(t_inlet=1.05, t_initial=1.0)
     grids, params, mesh = build_case(cfg, mesh)
     assert cfg.ddf_enabled and grids[-1].tau_g > 0.5
-    s, o = run_pair(cfg, grids, params, mesh, steps=8, strict=True)
+    s, o = run_pair(cfg, grids, params, mesh, steps=8, strict=True, storage=storage)
     res = compare(s, o, grids, fields=("f", "rho", "vel", "g", "T"))
     print(res)
     for k, (mx, nz, rel) in res.items():
