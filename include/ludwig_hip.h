@@ -258,6 +258,19 @@ int lb_engine_case_progress(const lb_engine* e, const lb_ctx* ctx, int64_t* t_do
 #define LB_SCHED_LINEAR 1 /* work-group i takes half-tile i               */
 int lb_engine_set_options(lb_engine* e, int32_t use_graph, int32_t schedule, int32_t waves);
 
+/* ------------------------------------------------------------------------
+ * Setup on the GPU.  compute_bouzidi_qmap_sparse (src/geometry/
+ * bouzidi_setup.jl:64-167, compute_q_for_cell bouzidi_math.jl:53-102): the
+ * same arguments and outputs as lbs_bouzidi_qmap (include/ludwig_setup.h),
+ * bit for bit (FP64, no contraction), on HIP device `device`.  The three
+ * arrays are malloc'd; release them with lb_free.  Errors: negative LB_E*,
+ * message in lb_setup_gpu_last_error().                                     */
+int lb_bouzidi_qmap_gpu(int device, int64_t nb, const int32_t* coords, const int32_t* grid, int gdx, int gdy,
+                        int gdz, const double* tris, int64_t ntri, double dx, const double* offset,
+                        int32_t** cells_out, uint16_t** q_out, int32_t** tri_out, int64_t* n_out);
+void lb_free(void* p);
+const char* lb_setup_gpu_last_error(void);
+
 /* HBM bandwidth probe (no counterpart in the reference): streams `bytes`
  * with 16-byte accesses; gbps_out[0] copy (read + write counted),
  * [1] read-only, [2] write-only, each averaged over `reps` launches.

@@ -1,6 +1,6 @@
 """Build the in-tree native libraries of openludwig_amd.
 
-libludwig_hip.so   gfx950 kernels + C ABI (include/ludwig_hip.h)
+libludwig_hip.so   gfx950 kernels + C ABI (include/ludwig_hip.h), GPU setup
 libludwig_setup.so host table producers (include/ludwig_setup.h)
 
 The kernel source is compiled twice: with ``-ffp-contract=off`` (namespace
@@ -59,8 +59,8 @@ def _newer(target: Path, deps: list[Path]) -> bool:
 def build_hip(force: bool = True) -> Path:
     OUT.mkdir(exist_ok=True)
     lib = OUT / "libludwig_hip.so"
-    deps = [CSRC / "lb_kernels.hip", CSRC / "lb_api.hip", CSRC / "lb_probe.hip", CSRC / "lb_kernels.h",
-            INC / "ludwig_hip.h"]
+    deps = [CSRC / "lb_kernels.hip", CSRC / "lb_api.hip", CSRC / "lb_probe.hip", CSRC / "lb_setup_gpu.hip",
+            CSRC / "lb_kernels.h", CSRC / "lb_geom.h", INC / "ludwig_hip.h"]
     if not force and _newer(lib, deps):
         return lib
     objs = []
@@ -69,7 +69,7 @@ def build_hip(force: bool = True) -> Path:
         _run([HIPCC, *DEVICE_FLAGS, f"-ffp-contract={contract}", f"-DLBK_NS={ns}", "-c",
               str(CSRC / "lb_kernels.hip"), "-o", str(obj)])
         objs.append(obj)
-    for src in ("lb_api", "lb_probe"):
+    for src in ("lb_api", "lb_probe", "lb_setup_gpu"):
         obj = OUT / f"{src}.o"
         _run([HIPCC, *DEVICE_FLAGS, "-ffp-contract=off", "-c", str(CSRC / f"{src}.hip"), "-o", str(obj)])
         objs.append(obj)
@@ -82,7 +82,7 @@ def build_hip(force: bool = True) -> Path:
 def build_setup(force: bool = True) -> Path:
     OUT.mkdir(exist_ok=True)
     lib = OUT / "libludwig_setup.so"
-    deps = [CSRC / "lb_setup.cpp", INC / "ludwig_setup.h"]
+    deps = [CSRC / "lb_setup.cpp", CSRC / "lb_geom.h", INC / "ludwig_setup.h"]
     if not force and _newer(lib, deps):
         return lib
     tmp = OUT / "libludwig_setup.so.tmp"

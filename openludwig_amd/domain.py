@@ -267,11 +267,45 @@ def should_use_bouzidi(level_id: int, num_levels: int, method: str, bouzidi_leve
     return method == "bouzidi" and level_id > num_levels - bouzidi_levels
 
 
+def bouzidi_qmap(coords: np.ndarray, dense_c: np.ndarray, dims: tuple, tris: np.ndarray, dx: float, off: np.ndarray,
+                 gpu_device: int | None = None) -> tuple:
+    """compute_bouzidi_qmap_sparse (bouzidi_setup.jl:64-167): (cells (n,4),
+    q bits (n,27), triangle ids (n,27)) in the canonical order, on the CPU
+    (lbs_bouzidi_qmap) or on a HIP device (lb_bouzidi_qmap_gpu)."""
+    nb = coords.shape[0]
+    ci = N.ptr(coords, C.c_int32)
+    gi = N.ptr(dense_c, C.c_int32)
+    cells, q, tri = N.i32p(), N.u16p(), N.i32p()
+    n = C.c_int64()
+    args = (nb, ci, gi, dims[0], dims[1], dims[2], N.ptr(tris, C.c_double), tris.shape[0], dx,
+            N.ptr(off, C.c_double), C.byref(cells), C.byref(q), C.byref(tri), C.byref(n))
+    if gpu_device is None:
+        lib = N.setup_lib()
+        N.check_setup(lib.lbs_bouzidi_qmap(*args), "lbs_bouzidi_qmap")
+        free = lib.lbs_free
+    else:
+        lib = N.hip_lib()
+        rc = lib.lb_bouzidi_qmap_gpu(int(gpu_device), *args)
+        if rc != 0:
+            raise RuntimeError(f"lb_bouzidi_qmap_gpu failed ({rc}): {lib.lb_setup_gpu_last_error().decode()}")
+        free = lib.lb_free
+    try:
+        m = n.value
+        out = (np.ctypeslib.as_array(cells, shape=(max(m, 1) * 4,))[:m * 4].copy().reshape(m, 4),
+               np.ctypeslib.as_array(q, shape=(max(m, 1) * 27,))[:m * 27].copy().reshape(m, 27),
+               np.ctypeslib.as_array(tri, shape=(max(m, 1) * 27,))[:m * 27].copy().reshape(m, 27))
+    finally:
+        free(cells); free(q); free(tri)
+    return out
+
+
 def setup_multilevel_domain(cfg: CaseConfig, mesh: SolverMesh, params: DomainParameters,
-                            verbose: bool = False, topology_only: bool = False) -> list:
+                            verbose: bool = False, topology_only: bool = False, gpu_device: int | None = None) -> list:
     """setup_multilevel_domain(mesh, params) (domain.jl:1-246).  With
     topology_only, only the block sets are built and the block count of each
-    level is returned (used to size serving requests)."""
+    level is returned (used to size serving requests).  With gpu_device, the
+    Bouzidi Q-map is computed on that HIP device (lb_bouzidi_qmap_gpu, the
+    same bits as the CPU restatement)."""
     lib = N.setup_lib()
     off = np.ascontiguousarray(params.mesh_offset, dtype=np.float64)
     placed_min = np.array(params.mesh_min) + off
@@ -352,18 +386,8 @@ def setup_multilevel_domain(cfg: CaseConfig, mesh: SolverMesh, params: DomainPar
                          np.zeros((0, 0, 0), np.int32),
                          grid_dims=dims, obstacle=obst, sponge=sponge, wall_dist=wall)
         if should_use_bouzidi(lvl, params.num_levels, cfg.boundary_method, cfg.bouzidi_levels):
-            cells, q, tri = N.i32p(), N.u16p(), N.i32p()
-            n = C.c_int64()
-            N.check_setup(lib.lbs_bouzidi_qmap(nb, ci, gi, dims[0], dims[1], dims[2], N.ptr(tris, C.c_double),
-                                               tris.shape[0], dx, N.ptr(off, C.c_double), C.byref(cells),
-                                               C.byref(q), C.byref(tri), C.byref(n)), "lbs_bouzidi_qmap")
-            try:
-                m = n.value
-                lev.bouzidi_cells = np.ctypeslib.as_array(cells, shape=(max(m, 1) * 4,))[:m * 4].copy().reshape(m, 4)
-                lev.bouzidi_q = np.ctypeslib.as_array(q, shape=(max(m, 1) * 27,))[:m * 27].copy().reshape(m, 27)
-                lev.bouzidi_tri = np.ctypeslib.as_array(tri, shape=(max(m, 1) * 27,))[:m * 27].copy().reshape(m, 27)
-            finally:
-                lib.lbs_free(cells); lib.lbs_free(q); lib.lbs_free(tri)
+            lev.bouzidi_cells, lev.bouzidi_q, lev.bouzidi_tri = bouzidi_qmap(
+                coords, dense_c, dims, tris, dx, off, gpu_device)
         if verbose:
             print(f"[setup] level {lvl}: {nb} blocks ({nb * 512 / 1e6:.2f} M cells), filled {filled}, "
                   f"bouzidi cells {lev.n_boundary_cells}")

@@ -8,8 +8,9 @@ a fixed amount of simulated time.  Two ways to serve a stream of requests:
 run_sequential   the reference's way (run_all_cases, main.jl:425-469): one
                  case at a time -- set up, run through execute_timestep_batch!
                  in batches of gpu_async_depth, compute forces, free.
-run_batched      continuous batching: set-up (CPU tables, then device upload)
-                 runs on worker threads as requests arrive; a case is
+run_batched      continuous batching: set-up (CPU tables with the Bouzidi
+                 Q-map on the GPU, then device upload) runs on worker threads
+                 as requests arrive; a case is
                  admitted when its exact device footprint (lb_footprint) fits
                  the HBM budget; admitted cases are attached to one Engine and
                  advance together in batched launches replayed as a hipGraph;
@@ -245,10 +246,13 @@ _TUNED: dict = {}
 
 def run_batched(requests: list, storage: int = N.LB_STORAGE_FP16, strict_fp: bool = False,
                 hbm_budget: int | None = None, reserve: int = 8 << 30, setup_workers: int = 4,
-                chunk: int = 4, options: dict | None = None, tune: bool = True, log=None) -> list:
+                chunk: int = 4, options: dict | None = None, tune: bool = True, log=None,
+                gpu_setup: bool = True) -> list:
     """Continuous batching over one Engine (see the module docstring).
     Launch options: `options` if given, else autotuned once per (storage,
-    deepest level count) on the first cases admitted, and reused."""
+    deepest level count) on the first cases admitted, and reused.  With
+    gpu_setup, the Bouzidi Q-map of each case is computed on the GPU (the same
+    bits as the CPU restatement the plain form uses)."""
     if hbm_budget is None:
         free, _total = device_memory()
         hbm_budget = free - reserve
@@ -265,7 +269,7 @@ def run_batched(requests: list, storage: int = N.LB_STORAGE_FP16, strict_fp: boo
         try:
             start = time.perf_counter() - t0
             cfg, mesh = request_case(req)
-            grids, params, mesh = build_case(cfg, mesh)
+            grids, params, mesh = build_case(cfg, mesh, gpu_device=0 if gpu_setup else None)
             n = req.coarse_steps(len(grids))
             nbytes = footprint(cfg, grids, params, mesh, storage, len(ramp_table(cfg, n)))
             budget.acquire(nbytes)
