@@ -22,6 +22,10 @@
 #include <limits>
 #include <vector>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include "ludwig_setup.h"
 #include "lb_geom.h"
 
@@ -262,58 +266,138 @@ int lbs_voxelize(int64_t nb, const int32_t* coords, const int32_t* grid, int gdx
 // perform_flood_fill!: src/geometry/domain_generation.jl:114-203.  Cells not
 // reachable (6-connectivity) from the non-solid cells of the min-bx blocks
 // become solid.  Returns the number of filled cells.
+// perform_flood_fill! (domain_generation.jl:114-203): the fluid cells 6-connected
+// to the fluid cells of the blocks with the smallest bx are kept; every other
+// fluid cell becomes solid.  The result is a set, so it is computed in
+// parallel: (1) per block, 6-connected components of its fluid cells; (2) a
+// graph of components joined across block faces; (3) reachability from the
+// seed components over that graph; (4) unreached fluid cells are filled.
 int64_t lbs_flood_fill(int64_t nb, const int32_t* coords, const int32_t* grid, int gdx, int gdy, int gdz,
                        uint8_t* obst) {
     if (nb == 0) return 0;
     Grid g{gdx, gdy, gdz, grid};
-    std::vector<uint8_t> vis((size_t)nb * 512, 0);
-    std::vector<int64_t> q;
-    q.reserve((size_t)nb * 64);
     int32_t minbx = coords[0];
     for (int64_t i = 0; i < nb; i++) minbx = std::min(minbx, coords[i * 3]);
-    for (int64_t b = 0; b < nb; b++)
-        if (coords[b * 3] == minbx)
-            for (int c = 0; c < 512; c++)
-                if (!obst[b * 512 + c]) {
-                    vis[b * 512 + c] = 1;
-                    q.push_back(b * 512 + c);
-                }
-    const int ddx[6] = {1, -1, 0, 0, 0, 0}, ddy[6] = {0, 0, 1, -1, 0, 0}, ddz[6] = {0, 0, 0, 0, 1, -1};
-    size_t head = 0;
-    while (head < q.size()) {
-        int64_t cur = q[head++];
-        int64_t b = cur / 512;
-        int c = (int)(cur % 512);
-        int lx = c % 8 + 1, ly = (c / 8) % 8 + 1, lz = c / 64 + 1;
-        int64_t bx = coords[b * 3], by = coords[b * 3 + 1], bz = coords[b * 3 + 2];
-        for (int i = 0; i < 6; i++) {
-            int nx = lx + ddx[i], ny = ly + ddy[i], nz = lz + ddz[i];
-            if (nx >= 1 && nx <= BS && ny >= 1 && ny <= BS && nz >= 1 && nz <= BS) {
-                int64_t id = b * 512 + (nx - 1) + 8 * (ny - 1) + 64 * (nz - 1);
-                if (!vis[id] && !obst[id]) {
-                    vis[id] = 1;
-                    q.push_back(id);
-                }
-            } else {
-                int32_t nbi = g.at(bx + ddx[i], by + ddy[i], bz + ddz[i]);
-                if (nbi > 0) {
-                    int wx = nx < 1 ? BS : (nx > BS ? 1 : nx);
-                    int wy = ny < 1 ? BS : (ny > BS ? 1 : ny);
-                    int wz = nz < 1 ? BS : (nz > BS ? 1 : nz);
-                    int64_t id = (int64_t)(nbi - 1) * 512 + (wx - 1) + 8 * (wy - 1) + 64 * (wz - 1);
-                    if (!vis[id] && !obst[id]) {
-                        vis[id] = 1;
-                        q.push_back(id);
+    // (1) components inside each block; comp[cell] = local id, -1 for solid
+    std::vector<int16_t> comp((size_t)nb * 512, -1);
+    std::vector<int32_t> ncomp((size_t)nb, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t b = 0; b < nb; b++) {
+        const uint8_t* o = obst + b * 512;
+        int16_t* cp = comp.data() + b * 512;
+        int16_t next = 0;
+        int16_t stack[512];
+        for (int c0 = 0; c0 < 512; c0++) {
+            if (o[c0] || cp[c0] >= 0) continue;
+            int sp = 0;
+            stack[sp++] = (int16_t)c0;
+            cp[c0] = next;
+            while (sp > 0) {
+                const int c = stack[--sp];
+                const int x = c & 7, y = (c >> 3) & 7, z = c >> 6;
+                const int nbr[6] = {x < 7 ? c + 1 : -1, x > 0 ? c - 1 : -1, y < 7 ? c + 8 : -1,
+                                    y > 0 ? c - 8 : -1, z < 7 ? c + 64 : -1, z > 0 ? c - 64 : -1};
+                for (int n : nbr)
+                    if (n >= 0 && !o[n] && cp[n] < 0) {
+                        cp[n] = next;
+                        stack[sp++] = (int16_t)n;
                     }
-                }
+            }
+            next++;
+        }
+        ncomp[(size_t)b] = next;
+    }
+    std::vector<int64_t> base((size_t)nb + 1, 0);
+    for (int64_t b = 0; b < nb; b++) base[(size_t)b + 1] = base[(size_t)b] + ncomp[(size_t)b];
+    const int64_t nc = base[(size_t)nb];
+    // (2) component graph: edges across the +x, +y, +z faces of every block
+    std::vector<std::vector<std::pair<int64_t, int64_t>>> edges_t;
+#pragma omp parallel
+    {
+#pragma omp single
+        {
+#ifdef _OPENMP
+            edges_t.resize((size_t)omp_get_num_threads());
+#else
+            edges_t.resize(1);
+#endif
+        }
+#ifdef _OPENMP
+        auto& my = edges_t[(size_t)omp_get_thread_num()];
+#else
+        auto& my = edges_t[0];
+#endif
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t b = 0; b < nb; b++) {
+            const int64_t bx = coords[b * 3], by = coords[b * 3 + 1], bz = coords[b * 3 + 2];
+            for (int axis = 0; axis < 3; axis++) {
+                const int32_t nbi = g.at(bx + (axis == 0), by + (axis == 1), bz + (axis == 2));
+                if (nbi <= 0) continue;
+                const int64_t nb1 = nbi - 1;
+                int64_t last_a = -1, last_b = -1;
+                for (int u = 0; u < 8; u++)
+                    for (int v = 0; v < 8; v++) {
+                        // cell on this block's + face and the facing cell of the neighbour
+                        int ca, cb;
+                        if (axis == 0) { ca = 7 + 8 * u + 64 * v; cb = 0 + 8 * u + 64 * v; }
+                        else if (axis == 1) { ca = u + 8 * 7 + 64 * v; cb = u + 0 + 64 * v; }
+                        else { ca = u + 8 * v + 64 * 7; cb = u + 8 * v; }
+                        const int16_t pa = comp[(size_t)b * 512 + ca], pb = comp[(size_t)nb1 * 512 + cb];
+                        if (pa < 0 || pb < 0) continue;
+                        const int64_t ga = base[(size_t)b] + pa, gb = base[(size_t)nb1] + pb;
+                        if (ga == last_a && gb == last_b) continue;
+                        last_a = ga;
+                        last_b = gb;
+                        my.emplace_back(ga, gb);
+                    }
             }
         }
     }
+    std::vector<int64_t> deg((size_t)nc + 1, 0);
+    for (auto& v : edges_t)
+        for (auto& e : v) {
+            deg[(size_t)e.first + 1]++;
+            deg[(size_t)e.second + 1]++;
+        }
+    for (int64_t i = 0; i < nc; i++) deg[(size_t)i + 1] += deg[(size_t)i];
+    std::vector<int64_t> adj((size_t)deg[(size_t)nc]);
+    {
+        std::vector<int64_t> fill(deg.begin(), deg.end() - 1);
+        for (auto& v : edges_t)
+            for (auto& e : v) {
+                adj[(size_t)fill[(size_t)e.first]++] = e.second;
+                adj[(size_t)fill[(size_t)e.second]++] = e.first;
+            }
+    }
+    // (3) reachability from the seed components (blocks with the smallest bx)
+    std::vector<uint8_t> reach((size_t)nc, 0);
+    std::vector<int64_t> q;
+    for (int64_t b = 0; b < nb; b++)
+        if (coords[b * 3] == minbx)
+            for (int32_t k = 0; k < ncomp[(size_t)b]; k++) {
+                reach[(size_t)(base[(size_t)b] + k)] = 1;
+                q.push_back(base[(size_t)b] + k);
+            }
+    for (size_t h = 0; h < q.size(); h++) {
+        const int64_t u = q[h];
+        for (int64_t i = deg[(size_t)u]; i < deg[(size_t)u + 1]; i++) {
+            const int64_t v = adj[(size_t)i];
+            if (!reach[(size_t)v]) {
+                reach[(size_t)v] = 1;
+                q.push_back(v);
+            }
+        }
+    }
+    // (4) fill the unreached fluid cells
     int64_t filled = 0;
-    for (int64_t i = 0; i < nb * 512; i++)
-        if (!obst[i] && !vis[i]) {
-            obst[i] = 1;
-            filled++;
+#pragma omp parallel for schedule(static) reduction(+ : filled)
+    for (int64_t b = 0; b < nb; b++)
+        for (int c = 0; c < 512; c++) {
+            const int16_t p = comp[(size_t)b * 512 + c];
+            if (p >= 0 && !reach[(size_t)(base[(size_t)b] + p)]) {
+                obst[b * 512 + c] = 1;
+                filled++;
+            }
         }
     return filled;
 }

@@ -369,3 +369,69 @@ def test_qmap_plane_known_answer():
             assert (qb[i][cxk == 1] == h03).all() and (qb[i][cxk != 1] == 0).all()
         else:
             assert (qb[i][cxk == -1] == h07).all() and (qb[i][cxk != -1] == 0).all()
+
+
+def _flood_fill_bfs(coords, dims, obst):
+    """Literal restatement of perform_flood_fill! (domain_generation.jl:114-203):
+    breadth-first search from the fluid cells of the blocks with the smallest
+    bx over 6-connected fluid cells; unreached fluid cells become solid."""
+    from collections import deque
+    nb = coords.shape[0]
+    index = {tuple(c): i for i, c in enumerate(coords)}
+    vis = np.zeros((nb, 512), dtype=bool)
+    q = deque()
+    minbx = coords[:, 0].min()
+    for b in range(nb):
+        if coords[b, 0] == minbx:
+            for c in range(512):
+                if not obst[b, c]:
+                    vis[b, c] = True
+                    q.append((b, c))
+    while q:
+        b, c = q.popleft()
+        x, y, z = c % 8, (c // 8) % 8, c // 64
+        for dx, dy, dz in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            nx, ny, nz = x + dx, y + dy, z + dz
+            bb = b
+            if not (0 <= nx < 8 and 0 <= ny < 8 and 0 <= nz < 8):
+                key = (coords[b, 0] + (nx > 7) - (nx < 0), coords[b, 1] + (ny > 7) - (ny < 0),
+                       coords[b, 2] + (nz > 7) - (nz < 0))
+                if key not in index:
+                    continue
+                bb = index[key]
+                nx, ny, nz = nx % 8, ny % 8, nz % 8
+            cc = nx + 8 * ny + 64 * nz
+            if not vis[bb, cc] and not obst[bb, cc]:
+                vis[bb, cc] = True
+                q.append((bb, cc))
+    out = obst.copy()
+    fill = (~vis) & (obst == 0)
+    out[fill] = 1
+    return out, int(fill.sum())
+
+
+@pytest.mark.parametrize("density", [0.0, 0.35, 0.6])
+def test_flood_fill_components_equal_bfs(density):
+    """The parallel component form of lbs_flood_fill equals the literal BFS on
+    random solid fields (many components per block) over a sparse block set."""
+    rng = np.random.default_rng(int(density * 100) + 3)
+    dims = (6, 5, 4)
+    active = rng.random(dims) < 0.7
+    active[0, :, :] = True
+    coords = np.ascontiguousarray(np.argwhere(active).astype(np.int32) + 1)
+    nb = coords.shape[0]
+    dense = np.zeros(dims, dtype=np.int32)
+    for i, c in enumerate(coords):
+        dense[c[0] - 1, c[1] - 1, c[2] - 1] = i + 1
+    dense_c = np.ascontiguousarray(np.asfortranarray(dense).ravel(order="F"))
+    obst = (rng.random((nb, 512)) < density).astype(np.uint8)
+    # a closed solid shell with fluid inside, away from the seed blocks
+    obst[nb // 2, :] = 1
+    obst[nb // 2, 73] = 0
+    ref, nref = _flood_fill_bfs(coords, dims, obst)
+    got = obst.copy()
+    lib = N.setup_lib()
+    n = lib.lbs_flood_fill(nb, N.ptr(coords, C.c_int32), N.ptr(dense_c, C.c_int32), dims[0], dims[1], dims[2],
+                           N.ptr(got, C.c_uint8))
+    assert n == nref
+    assert np.array_equal(got, ref)
