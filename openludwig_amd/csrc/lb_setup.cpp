@@ -446,9 +446,27 @@ int64_t lbs_wall_distances(int64_t nb, const int32_t* coords, const int32_t* gri
     Grid g{gdx, gdy, gdz, grid};
     float dxf = (float)dx;
     int64_t count = 0;
+    // a block whose own cells and 26 neighbour blocks hold no solid cell has
+    // no near-wall cell: skip it
+    std::vector<uint8_t> solid((size_t)nb, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t b = 0; b < nb; b++)
+        for (int c = 0; c < 512; c++)
+            if (obst[b * 512 + c]) {
+                solid[(size_t)b] = 1;
+                break;
+            }
 #pragma omp parallel for schedule(static) reduction(+ : count)
     for (int64_t b = 0; b < nb; b++) {
         int64_t bx = coords[b * 3], by = coords[b * 3 + 1], bz = coords[b * 3 + 2];
+        bool any = false;
+        for (int dz = -1; dz <= 1 && !any; dz++)
+            for (int dy = -1; dy <= 1 && !any; dy++)
+                for (int dxo = -1; dxo <= 1 && !any; dxo++) {
+                    const int32_t id = (dxo == 0 && dy == 0 && dz == 0) ? (int32_t)(b + 1) : g.at(bx + dxo, by + dy, bz + dz);
+                    if (id > 0 && solid[(size_t)id - 1]) any = true;
+                }
+        if (!any) continue;
         for (int lz = 1; lz <= BS; lz++)
             for (int ly = 1; ly <= BS; ly++)
                 for (int lx = 1; lx <= BS; lx++) {
