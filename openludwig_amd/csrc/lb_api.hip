@@ -1298,7 +1298,9 @@ int lb_engine_step(lb_engine* e, int32_t n) {
     if (e->cases.empty() || n == 0) return LB_OK;
     if (e->dirty) RC(engine_rebuild(e));
     if (e->use_graph && !e->exec) {
-        EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        // relaxed: other host threads (case set-up) may allocate and copy
+        // while the step is being captured
+        EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
         const hipError_t r = engine_issue_step(e);
         hipGraph_t g = nullptr;
         const hipError_t r2 = hipStreamEndCapture(e->stream, &g);

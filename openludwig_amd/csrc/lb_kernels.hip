@@ -258,6 +258,11 @@ struct Codec<0> {
         return *(const GF*)((const GC*)base + off);
     }
     static __device__ __forceinline__ void store(void* base, size_t i, float f, float) { ((GF*)base)[i] = f; }
+    // non-temporal store, for the stream-collide outputs (read again only
+    // one substep later, after the whole level has streamed through L2)
+    static __device__ __forceinline__ void stream(void* base, size_t i, float f, float) {
+        __builtin_nontemporal_store(f, &((GF*)base)[i]);
+    }
 };
 template <>
 struct Codec<1> {
@@ -272,6 +277,9 @@ struct Codec<1> {
     }
     static __device__ __forceinline__ void store(void* base, size_t i, float f, float w) {
         ((GU16*)base)[i] = __half_as_ushort(__float2half_rn(f - w));
+    }
+    static __device__ __forceinline__ void stream(void* base, size_t i, float f, float w) {
+        __builtin_nontemporal_store((uint16_t)__half_as_ushort(__float2half_rn(f - w)), &((GU16*)base)[i]);
     }
 };
 template <>
@@ -293,6 +301,9 @@ struct Codec<2> {
     }
     static __device__ __forceinline__ void store(void* base, size_t i, float f, float w) {
         ((GU16*)base)[i] = enc(f - w);
+    }
+    static __device__ __forceinline__ void stream(void* base, size_t i, float f, float w) {
+        __builtin_nontemporal_store(enc(f - w), &((GU16*)base)[i]);
     }
 };
 
@@ -525,7 +536,7 @@ collide_kernel(const LevelArgs* __restrict__ las,
     const int gy = a.borg[b * 3 + 1] + y + 1;
     const int gz = a.borg[b * 3 + 2] + z + 1;
     const size_t i4 = (size_t)b * NC + c;
-    const bool is_obs = gp(a.obst)[i4] != 0;
+    const bool is_obs = __builtin_nontemporal_load(&gp(a.obst)[i4]) != 0;
 
     // STEP 1: pull (kernel_cumulant.jl:86-168).  All 27 loads are issued
     // before any is used, each one table read and one add.
@@ -560,8 +571,9 @@ collide_kernel(const LevelArgs* __restrict__ las,
         }
     }
     // lanes whose pull crosses into a missing block (domain boundary)
-    const float sp = gp(a.sponge)[i4];
-    const float wdist = a.wall_model == 1 ? gp(a.wall_dist)[i4] : 0.0f;
+    // single-use inputs: non-temporal loads keep L2 for the neighbour pulls
+    const float sp = __builtin_nontemporal_load(&gp(a.sponge)[i4]);
+    const float wdist = a.wall_model == 1 ? __builtin_nontemporal_load(&gp(a.wall_dist)[i4]) : 0.0f;
     if constexpr (BND) {
         uint32_t miss = 0u;
         const uint32_t m = wave_miss(smask, wv);
@@ -613,7 +625,7 @@ collide_kernel(const LevelArgs* __restrict__ las,
         vo[iv + 2 * NC] = 0.0f;
         ro[i4] = 1.0f;
 #pragma unroll
-        for (int k = 0; k < 27; k++) Codec<ST>::store(fo, fidx(b, k, c), A[OPP(k)], WK(k));
+        for (int k = 0; k < 27; k++) Codec<ST>::stream(fo, fidx(b, k, c), A[OPP(k)], WK(k));
         return;
     }
 
@@ -669,10 +681,10 @@ collide_kernel(const LevelArgs* __restrict__ las,
     const float uze = uz + 0.5f * Fz * inv_rho;
     {
         const size_t iv = (size_t)b * 3 * NC + c;
-        vo[iv] = ux;
-        vo[iv + NC] = uy;
-        vo[iv + 2 * NC] = uz;
-        ro[i4] = rho;
+        __builtin_nontemporal_store(ux, &vo[iv]);
+        __builtin_nontemporal_store(uy, &vo[iv + NC]);
+        __builtin_nontemporal_store(uz, &vo[iv + 2 * NC]);
+        __builtin_nontemporal_store(rho, &ro[i4]);
     }
 
     // STEP 3: WALE on the previous velocity (kernel_cumulant.jl:264-310)
@@ -876,11 +888,11 @@ collide_kernel(const LevelArgs* __restrict__ las,
             float ft = WK(k) * 3.0f *
                        ((cxf - ux + 3.0f * cu * cxf) * Fx + (cyf - uy + 3.0f * cu * cyf) * Fy +
                         (czf - uz + 3.0f * cu * czf) * Fz);
-            Codec<ST>::store(fo, fidx(b, k, c), A[k] + pre * ft, WK(k));
+            Codec<ST>::stream(fo, fidx(b, k, c), A[k] + pre * ft, WK(k));
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < 27; k++) Codec<ST>::store(fo, fidx(b, k, c), A[k], WK(k));
+        for (int k = 0; k < 27; k++) Codec<ST>::stream(fo, fidx(b, k, c), A[k], WK(k));
     }
 }
 
@@ -1005,7 +1017,7 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
                 float gw = geq(a.T_wall, 0.0f, 0.0f, 0.0f, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k));
                 go = 2.0f * gw - gopp;
             }
-            Codec<ST>::store(a.g_out, fidx(b, k, c), go, WK(k));
+            Codec<ST>::stream(a.g_out, fidx(b, k, c), go, WK(k));
         }
         a.T_out[i4] = a.adiabatic == 1 ? Tsum : a.T_wall;
         return;
@@ -1028,7 +1040,7 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
             gs = gs * (1.0f - sp) + gt * sp;
         }
         float ge = geq(Tl, ux, uy, uz, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k));
-        Codec<ST>::store(a.g_out, fidx(b, k, c), gs - omg * (gs - ge), WK(k));
+        Codec<ST>::stream(a.g_out, fidx(b, k, c), gs - omg * (gs - ge), WK(k));
     }
 }
 
