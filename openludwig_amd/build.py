@@ -60,7 +60,7 @@ def build_hip(force: bool = True) -> Path:
     OUT.mkdir(exist_ok=True)
     lib = OUT / "libludwig_hip.so"
     deps = [CSRC / "lb_kernels.hip", CSRC / "lb_api.hip", CSRC / "lb_probe.hip", CSRC / "lb_setup_gpu.hip",
-            CSRC / "lb_kernels.h", CSRC / "lb_geom.h", INC / "ludwig_hip.h"]
+            CSRC / "lb_kernels.h", CSRC / "lb_geom.h", CSRC / "lb_sync.h", INC / "ludwig_hip.h"]
     if not force and _newer(lib, deps):
         return lib
     objs = []

@@ -24,12 +24,14 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
 
 #include "ludwig_hip.h"
 #include "lb_kernels.h"
+#include "lb_sync.h"
 
 namespace {
 
@@ -423,7 +425,9 @@ int step_level(lb_ctx* c, int li, int64_t j) {
     const lbk::LevelArgs* las = c->las_dev;
     const lbk::CaseState* st = c->st_dev;
     if (L.nb > 0) {
-        if (li > 0 && L.n_gchunks > 0) HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 0, c->stream));
+        // one ghost pass per parent step fills the inputs of both child substeps
+        const bool fill = li > 0 && L.n_gchunks > 0 && (j & 1) == 0;
+        if (fill) HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 0, c->stream));
         hipEvent_t e1 = nullptr;
         RC(profile_begin(c, L, 0, &e1));
         HIPCHK(c, c->k->collide(las, st, L.wcol, L.n_int, j, storage, L.wide ? 1 : 0, 0, 0, 5, c->stream));
@@ -432,8 +436,7 @@ int step_level(lb_ctx* c, int li, int64_t j) {
         if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
         if (L.n_lchunks > 0) HIPCHK(c, c->k->bouzidi(las, st, L.wlk, L.n_lchunks, j, storage, 0, c->stream));
         if (ddf) {
-            if (li > 0 && L.n_gchunks > 0)
-                HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 1, c->stream));
+            if (fill) HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 1, c->stream));
             hipEvent_t e2 = nullptr;
             RC(profile_begin(c, L, 1, &e2));
             HIPCHK(c, c->k->thermal(las, st, L.wcol, L.nb, j, storage, L.wide_g ? 1 : 0, 0, c->stream));
@@ -682,13 +685,15 @@ hipError_t engine_op(lb_engine* e, int d, int64_t j) {
         if (r != hipSuccess) return r; \
     } while (0)
     const lbk::CaseState* st = e->states;
-    if (W.ngh) ETRY(k->ghost_fill(e->las, st, W.gh, W.ngh, j, e->storage, 0, e->stream));
+    // one ghost pass per parent step fills the inputs of both child substeps
+    const bool fill = (j & 1) == 0;
+    if (W.ngh && fill) ETRY(k->ghost_fill(e->las, st, W.gh, W.ngh, j, e->storage, 0, e->stream));
     if (W.ncol) ETRY(k->collide(e->las, st, W.col, W.ncol, j, e->storage, W.wide, 0, e->sched, e->waves, e->stream));
     if (W.ncolb)
         ETRY(k->collide(e->las, st, W.colb, W.ncolb, j, e->storage, W.wide, 1, e->sched, e->waves, e->stream));
     if (W.nlk) ETRY(k->bouzidi(e->las, st, W.lk, W.nlk, j, e->storage, 0, e->stream));
     if (W.ntcol) {
-        if (W.ntgh) ETRY(k->ghost_fill(e->las, st, W.tgh, W.ntgh, j, e->storage, 1, e->stream));
+        if (W.ntgh && fill) ETRY(k->ghost_fill(e->las, st, W.tgh, W.ntgh, j, e->storage, 1, e->stream));
         ETRY(k->thermal(e->las, st, W.tcol, W.ntcol, j, e->storage, W.wide_g, e->sched, e->stream));
         if (W.ntlk) ETRY(k->bouzidi(e->las, st, W.tlk, W.ntlk, j, e->storage, 1, e->stream));
     }
@@ -716,6 +721,13 @@ int check_ctx(lb_ctx* c) {
 
 }  // namespace
 
+namespace lbsync {
+std::shared_mutex& capture_mutex() {
+    static std::shared_mutex m;
+    return m;
+}
+}  // namespace lbsync
+
 extern "C" {
 
 const char* lb_last_error(const lb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -723,6 +735,7 @@ const char* lb_last_error(const lb_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 const char* lb_version(void) { return "ludwig_hip 0.1 gfx950"; }
 
 int lb_create(lb_ctx** out, int device, const lb_config* cfg) {
+    lbsync::SetupGuard guard;
     if (!out || !cfg) return LB_EINVAL;
     *out = nullptr;
     if (cfg->n_levels < 1 || cfg->n_levels > 16 || cfg->domain_nx < 8 || cfg->domain_ny < 8 || cfg->domain_nz < 8)
@@ -755,6 +768,7 @@ int lb_create(lb_ctx** out, int device, const lb_config* cfg) {
 }
 
 void lb_destroy(lb_ctx* c) {
+    lbsync::SetupGuard guard;
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -770,6 +784,7 @@ void lb_destroy(lb_ctx* c) {
 }
 
 int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
+    lbsync::SetupGuard guard;
     if (!c || !d) return LB_EINVAL;
     if (d->level_id < 1 || d->level_id > (int)c->levels.size())
         return fail(c, LB_EINVAL, "level_id %d out of range", d->level_id);
@@ -939,6 +954,7 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
 }
 
 int lb_init_equilibrium(lb_ctx* c, float T0) {
+    lbsync::SetupGuard guard;
     RC(check_ctx(c));
     HIPCHK(c, hipSetDevice(c->device));
     for (auto& L : c->levels) {
@@ -953,6 +969,7 @@ int lb_init_equilibrium(lb_ctx* c, float T0) {
 }
 
 int lb_step_batch(lb_ctx* c, int64_t t_start, int32_t n_steps, float u_curr, const lb_step_params* p) {
+    lbsync::SetupGuard guard;
     RC(check_ctx(c));
     if (!p || n_steps < 0 || t_start < 0) return fail(c, LB_EINVAL, "bad step arguments");
     HIPCHK(c, hipSetDevice(c->device));
@@ -965,6 +982,7 @@ int lb_step_batch(lb_ctx* c, int64_t t_start, int32_t n_steps, float u_curr, con
 }
 
 int lb_synchronize(lb_ctx* c) {
+    lbsync::SetupGuard guard;
     if (!c) return LB_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -973,6 +991,7 @@ int lb_synchronize(lb_ctx* c) {
 
 int lb_mesh_upload(lb_ctx* c, int64_t n, const float* cx, const float* cy, const float* cz, const float* nx,
                    const float* ny, const float* nz, const float* areas) {
+    lbsync::SetupGuard guard;
     if (!c || n < 0 || (!c->dry && n > 0 && (!cx || !cy || !cz || !nx || !ny || !nz || !areas))) return LB_EINVAL;
     if (c->n_tri != 0) return fail(c, LB_ESTATE, "mesh already uploaded");
     if (!c->dry) HIPCHK(c, hipSetDevice(c->device));
@@ -990,6 +1009,7 @@ int lb_mesh_upload(lb_ctx* c, int64_t n, const float* cx, const float* cy, const
 
 int lb_forces(lb_ctx* c, const lb_force_params* fp, double* out, float* p_map, float* sx_map, float* sy_map,
               float* sz_map) {
+    lbsync::SetupGuard guard;
     RC(check_ctx(c));
     if (!fp || !out) return fail(c, LB_EINVAL, "null argument");
     if (c->n_tri <= 0) return fail(c, LB_ESTATE, "no mesh uploaded");
@@ -1068,6 +1088,7 @@ int lb_forces(lb_ctx* c, const lb_force_params* fp, double* out, float* p_map, f
 }
 
 int lb_download_level(lb_ctx* c, int level_id, int field, float* dst) {
+    lbsync::SetupGuard guard;
     RC(check_ctx(c));
     if (!dst || level_id < 1 || level_id > (int)c->levels.size()) return fail(c, LB_EINVAL, "bad arguments");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1113,6 +1134,7 @@ int lb_download_level(lb_ctx* c, int level_id, int field, float* dst) {
 // Overwrite the current state of a level (both halves of the ping-pong pair,
 // so the next step reads it whatever its parity).
 int lb_upload_level(lb_ctx* c, int level_id, int field, const float* src) {
+    lbsync::SetupGuard guard;
     RC(check_ctx(c));
     if (!src || level_id < 1 || level_id > (int)c->levels.size()) return fail(c, LB_EINVAL, "bad arguments");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1161,6 +1183,7 @@ int lb_upload_level(lb_ctx* c, int level_id, int field, const float* src) {
 }
 
 int lb_stats(lb_ctx* c, int level_id, double* out) {
+    lbsync::SetupGuard guard;
     RC(check_ctx(c));
     if (!out || level_id < 1 || level_id > (int)c->levels.size()) return fail(c, LB_EINVAL, "bad arguments");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1202,6 +1225,7 @@ int lb_profile_enable(lb_ctx* c, int enable) {
 }
 
 int lb_profile_read_kernel(lb_ctx* c, int level_id, int kernel, double* ms, int64_t* launches) {
+    lbsync::SetupGuard guard;
     if (!c || !ms || !launches || level_id < 1 || level_id > (int)c->levels.size() || kernel < 0 || kernel > 1)
         return LB_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
@@ -1224,6 +1248,7 @@ int lb_profile_read(lb_ctx* c, int level_id, double* ms, int64_t* launches) {
 }
 
 int lb_engine_create(lb_engine** out, int device, int32_t storage, int32_t strict_fp) {
+    lbsync::SetupGuard guard;
     if (!out) return LB_EINVAL;
     *out = nullptr;
     if (storage < LB_STORAGE_FP32 || storage > LB_STORAGE_BF16) return LB_EINVAL;
@@ -1240,6 +1265,7 @@ int lb_engine_create(lb_engine** out, int device, int32_t storage, int32_t stric
 }
 
 void lb_engine_destroy(lb_engine* e) {
+    lbsync::SetupGuard guard;
     if (!e) return;
     (void)hipSetDevice(e->device);
     (void)hipStreamSynchronize(e->stream);
@@ -1253,6 +1279,7 @@ const char* lb_engine_last_error(const lb_engine* e) { return e ? e->err.c_str()
 
 int lb_engine_add(lb_engine* e, lb_ctx* c, int64_t n_coarse_steps, const float* u_batches, int32_t n_batches,
                   int32_t batch_len, const lb_step_params* p) {
+    lbsync::SetupGuard guard;
     if (!e || !c || !u_batches || !p) return LB_EINVAL;
     RC(check_ctx(c));
     if (c->device != e->device) return efail(e, LB_EINVAL, "case is on device %d, engine on %d", c->device, e->device);
@@ -1277,6 +1304,7 @@ int lb_engine_add(lb_engine* e, lb_ctx* c, int64_t n_coarse_steps, const float* 
 }
 
 int lb_engine_remove(lb_engine* e, lb_ctx* c) {
+    lbsync::SetupGuard guard;
     if (!e || !c) return LB_EINVAL;
     EHIPCHK(e, hipSetDevice(e->device));
     for (size_t i = 0; i < e->cases.size(); i++) {
@@ -1296,10 +1324,14 @@ int lb_engine_step(lb_engine* e, int32_t n) {
     if (!e || n < 0) return LB_EINVAL;
     EHIPCHK(e, hipSetDevice(e->device));
     if (e->cases.empty() || n == 0) return LB_OK;
-    if (e->dirty) RC(engine_rebuild(e));
+    if (e->dirty) {
+        lbsync::SetupGuard guard;
+        RC(engine_rebuild(e));
+    }
     if (e->use_graph && !e->exec) {
-        // relaxed: other host threads (case set-up) may allocate and copy
-        // while the step is being captured
+        // GPU calls of other threads wait for the capture (lb_sync.h);
+        // relaxed mode in addition lets them run host-side work meanwhile
+        std::unique_lock<std::shared_mutex> capture(lbsync::capture_mutex());
         EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
         const hipError_t r = engine_issue_step(e);
         hipGraph_t g = nullptr;
@@ -1321,6 +1353,7 @@ int lb_engine_step(lb_engine* e, int32_t n) {
 }
 
 int lb_engine_synchronize(lb_engine* e) {
+    lbsync::SetupGuard guard;
     if (!e) return LB_EINVAL;
     EHIPCHK(e, hipSetDevice(e->device));
     EHIPCHK(e, hipStreamSynchronize(e->stream));
@@ -1377,6 +1410,7 @@ int lb_footprint(const lb_config* cfg, const lb_level_desc* levels, int32_t n_le
 }
 
 int lb_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes) {
+    lbsync::SetupGuard guard;
     if (!free_bytes || !total_bytes) return LB_EINVAL;
     if (hipSetDevice(device) != hipSuccess) return LB_EHIP;
     size_t f = 0, t = 0;

@@ -90,7 +90,8 @@ struct ThermalArgs {
 };
 
 struct GhostArgs {
-    void* f;  // populations being pulled (ghost blocks at index >= nb)
+    void* f;   // input half of substep j (ghost blocks at index >= nb)
+    void* f2;  // input half of substep j + 1
     const int32_t* entries;
     int64_t n;
     const int32_t* borg;
@@ -102,7 +103,7 @@ struct GhostArgs {
     const float* u_old;
     const int32_t* pgrid;
     int32_t pdx, pdy, pdz;
-    float tau_fine, tau_coarse, tw;
+    float tau_fine, tau_coarse;
     int32_t temporal, comp, thermal;
 };
 
@@ -201,16 +202,18 @@ __device__ __forceinline__ ThermalArgs thermal_view(const LevelArgs& L, const Ca
     return t;
 }
 
-// Coarse-to-fine source of substep j (level > 1): the parent's last step is
-// substep j >> 1 of the parent level; its output pair half is "new", its
-// input half (kept untouched until the parent's next step) is "old".  The
-// temporal weight is 0 for the first child substep and 0.5 for the second
-// (solver_control.jl:66-96).
+// Coarse-to-fine source of substeps j and j + 1 (j even, level > 1): the
+// parent's last step is substep j >> 1 of the parent level; its output pair
+// half is "new", its input half (kept untouched until the parent's next
+// step) is "old".  The temporal weight is 0 for the first child substep and
+// 0.5 for the second (solver_control.jl:66-96); f and f2 are their input
+// halves.
 __device__ __forceinline__ GhostArgs ghost_view(const LevelArgs& L, const CaseState& S, int64_t j, int thermal) {
     const int par = (int)(substep(L, S, j) & 1);
     const int ppar = (int)(((S.t0 << (L.lshift - 1)) + (j >> 1)) & 1);
     GhostArgs g;
     g.f = thermal ? (void*)L.g[par] : L.f[par];
+    g.f2 = thermal ? (void*)L.g[1 - par] : L.f[1 - par];
     g.entries = L.gent;
     g.n = L.ngent;
     g.borg = L.borg;
@@ -226,7 +229,6 @@ __device__ __forceinline__ GhostArgs ghost_view(const LevelArgs& L, const CaseSt
     g.pdz = L.pdz;
     g.tau_fine = thermal ? L.tau_g : L.tau;
     g.tau_coarse = thermal ? L.ptau_g : L.ptau;
-    g.tw = (j & 1) ? 0.5f : 0.0f;
     g.temporal = S.temporal;
     g.comp = S.comp;
     g.thermal = thermal;
@@ -1063,8 +1065,13 @@ hipError_t thermal(const LevelArgs* las, const CaseState* states, const Work* wo
 // ---------------------------------------------------------------------------
 // Ghost-cell interpolation: interpolate_with_rescaling / its thermal twin
 // (physics_interpolation.jl:4-127, 135-248) for every direction a ghost cell
-// serves.  The corner geometry and the density/velocity part of the
-// trilinear blend depend on the cell only and are computed once; each
+// serves.  The two substeps of a child level within one parent step read the
+// same parent state and differ only in the temporal weight (0, then 0.5), so
+// one pass, run before the first substep, computes both: the tw = 0 values go
+// to the ghost slots of the first substep's input half, the tw = 0.5 values to
+// the other half, which the first substep does not write (its collide writes
+// real blocks only).  The corner geometry and the density/velocity part of
+// the trilinear blend depend on the cell only and are computed once; each
 // direction then blends its own population component.  Per value, the
 // operations are those of the reference function, in its order.
 template <int ST>
@@ -1093,58 +1100,89 @@ __global__ void __launch_bounds__(256) ghost_kernel(const LevelArgs* __restrict_
     px0 = max(1, px0);
     py0 = max(1, py0);
     pz0 = max(1, pz0);
-    const bool use_old = a.temporal == 1 && a.tw < 0.99f;
-    const float tw = a.tw;
+    // temporal blend with tw = 0 (substep j) and tw = 0.5 (substep j + 1)
+    const bool use_old0 = a.temporal == 1 && 0.0f < 0.99f;
+    const bool use_old1 = a.temporal == 1 && 0.5f < 0.99f;
+    const float tw0 = 0.0f, tw1 = 0.5f;
     int cb[8], cc[8];
     bool ok[8];
-    float v[8][4];
+    float v0[8][4], v1[8][4];
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const int pgx = (j & 1) ? px1 : px0;
-        const int pgy = (j & 2) ? py1 : py0;
-        const int pgz = (j & 4) ? pz1 : pz0;
+    for (int q = 0; q < 8; q++) {
+        const int pgx = (q & 1) ? px1 : px0;
+        const int pgy = (q & 2) ? py1 : py0;
+        const int pgz = (q & 4) ? pz1 : pz0;
         const int pbx = (pgx - 1) / BS + 1, pby = (pgy - 1) / BS + 1, pbz = (pgz - 1) / BS + 1;
-        ok[j] = false;
-        cb[j] = 0;
-        cc[j] = 0;
+        ok[q] = false;
+        cb[q] = 0;
+        cc[q] = 0;
         if (pbx >= 1 && pbx <= a.pdx && pby >= 1 && pby <= a.pdy && pbz >= 1 && pbz <= a.pdz) {
             const int pb = a.pgrid[(pbx - 1) + a.pdx * ((pby - 1) + a.pdy * (pbz - 1))];
             if (pb >= 0) {
-                ok[j] = true;
-                cb[j] = pb;
-                cc[j] = ((pgx - 1) % BS) + BS * ((pgy - 1) % BS) + BS * BS * ((pgz - 1) % BS);
+                ok[q] = true;
+                cb[q] = pb;
+                cc[q] = ((pgx - 1) % BS) + BS * ((pgy - 1) % BS) + BS * BS * ((pgz - 1) % BS);
             }
         }
-        if (ok[j]) {
-            const size_t i4 = (size_t)cb[j] * NC + cc[j];
-            const size_t iv = (size_t)cb[j] * 3 * NC + cc[j];
+        if (ok[q]) {
+            const size_t i4 = (size_t)cb[q] * NC + cc[q];
+            const size_t iv = (size_t)cb[q] * 3 * NC + cc[q];
             const float rn = a.s_new[i4], uxn = a.u_new[iv], uyn = a.u_new[iv + NC], uzn = a.u_new[iv + 2 * NC];
-            if (use_old) {
-                const float ro = a.s_old[i4], uxo = a.u_old[iv], uyo = a.u_old[iv + NC], uzo = a.u_old[iv + 2 * NC];
-                v[j][0] = ro * (1.0f - tw) + rn * tw;
-                v[j][1] = uxo * (1.0f - tw) + uxn * tw;
-                v[j][2] = uyo * (1.0f - tw) + uyn * tw;
-                v[j][3] = uzo * (1.0f - tw) + uzn * tw;
-            } else {
-                v[j][0] = rn; v[j][1] = uxn; v[j][2] = uyn; v[j][3] = uzn;
+            float ro = 0.0f, uxo = 0.0f, uyo = 0.0f, uzo = 0.0f;
+            if (use_old0 || use_old1) {
+                ro = a.s_old[i4];
+                uxo = a.u_old[iv];
+                uyo = a.u_old[iv + NC];
+                uzo = a.u_old[iv + 2 * NC];
             }
-        } else if (j == 0) {
-            v[0][0] = 1.0f; v[0][1] = 0.0f; v[0][2] = 0.0f; v[0][3] = 0.0f;
+            if (use_old0) {
+                v0[q][0] = ro * (1.0f - tw0) + rn * tw0;
+                v0[q][1] = uxo * (1.0f - tw0) + uxn * tw0;
+                v0[q][2] = uyo * (1.0f - tw0) + uyn * tw0;
+                v0[q][3] = uzo * (1.0f - tw0) + uzn * tw0;
+            } else {
+                v0[q][0] = rn; v0[q][1] = uxn; v0[q][2] = uyn; v0[q][3] = uzn;
+            }
+            if (use_old1) {
+                v1[q][0] = ro * (1.0f - tw1) + rn * tw1;
+                v1[q][1] = uxo * (1.0f - tw1) + uxn * tw1;
+                v1[q][2] = uyo * (1.0f - tw1) + uyn * tw1;
+                v1[q][3] = uzo * (1.0f - tw1) + uzn * tw1;
+            } else {
+                v1[q][0] = rn; v1[q][1] = uxn; v1[q][2] = uyn; v1[q][3] = uzn;
+            }
+        } else if (q == 0) {
+            v0[0][0] = 1.0f; v0[0][1] = 0.0f; v0[0][2] = 0.0f; v0[0][3] = 0.0f;
+            v1[0][0] = 1.0f; v1[0][1] = 0.0f; v1[0][2] = 0.0f; v1[0][3] = 0.0f;
         } else {
 #pragma unroll
-            for (int q = 0; q < 4; q++) v[j][q] = v[0][q];
+            for (int m = 0; m < 4; m++) {
+                v0[q][m] = v0[0][m];
+                v1[q][m] = v1[0][m];
+            }
         }
     }
-    float r[4];
+    float r0[4], r1[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        float c00 = v[0][q] * (1.0f - wx) + v[1][q] * wx;
-        float c01 = v[4][q] * (1.0f - wx) + v[5][q] * wx;
-        float c10 = v[2][q] * (1.0f - wx) + v[3][q] * wx;
-        float c11 = v[6][q] * (1.0f - wx) + v[7][q] * wx;
-        float c0 = c00 * (1.0f - wy) + c10 * wy;
-        float c1 = c01 * (1.0f - wy) + c11 * wy;
-        r[q] = c0 * (1.0f - wz) + c1 * wz;
+    for (int m = 0; m < 4; m++) {
+        {
+            float c00 = v0[0][m] * (1.0f - wx) + v0[1][m] * wx;
+            float c01 = v0[4][m] * (1.0f - wx) + v0[5][m] * wx;
+            float c10 = v0[2][m] * (1.0f - wx) + v0[3][m] * wx;
+            float c11 = v0[6][m] * (1.0f - wx) + v0[7][m] * wx;
+            float c0 = c00 * (1.0f - wy) + c10 * wy;
+            float c1 = c01 * (1.0f - wy) + c11 * wy;
+            r0[m] = c0 * (1.0f - wz) + c1 * wz;
+        }
+        {
+            float c00 = v1[0][m] * (1.0f - wx) + v1[1][m] * wx;
+            float c01 = v1[4][m] * (1.0f - wx) + v1[5][m] * wx;
+            float c10 = v1[2][m] * (1.0f - wx) + v1[3][m] * wx;
+            float c11 = v1[6][m] * (1.0f - wx) + v1[7][m] * wx;
+            float c0 = c00 * (1.0f - wy) + c10 * wy;
+            float c1 = c01 * (1.0f - wy) + c11 * wy;
+            r1[m] = c0 * (1.0f - wz) + c1 * wz;
+        }
     }
     const float tc = a.tau_coarse - 0.5f;
     const float tf = a.tau_fine - 0.5f;
@@ -1154,36 +1192,51 @@ __global__ void __launch_bounds__(256) ghost_kernel(const LevelArgs* __restrict_
         scale = scale > 100.0f ? 100.0f : (scale < 0.01f ? 0.01f : scale);
     }
     void* const fo = a.f;
+    void* const fo2 = a.f2;
 #pragma unroll 1
     for (int k = 0; k < 27; k++) {
         if (!(mask & (1u << k))) continue;
         const float w_k = wk_rt(k), cx = (float)CX(k), cy = (float)CY(k), cz = (float)CZ(k);
-        float fv[8];
+        float f0[8], f1[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            if (ok[j]) {
-                const float fn = Codec<ST>::load(a.a_new, fidx(cb[j], k, cc[j]), w_k);
-                if (use_old) {
-                    const float fo_ = Codec<ST>::load(a.a_old, fidx(cb[j], k, cc[j]), w_k);
-                    fv[j] = fo_ * (1.0f - tw) + fn * tw;
-                } else {
-                    fv[j] = fn;
-                }
+        for (int q = 0; q < 8; q++) {
+            if (ok[q]) {
+                const float fn = Codec<ST>::load(a.a_new, fidx(cb[q], k, cc[q]), w_k);
+                float fo_ = 0.0f;
+                if (use_old0 || use_old1) fo_ = Codec<ST>::load(a.a_old, fidx(cb[q], k, cc[q]), w_k);
+                f0[q] = use_old0 ? fo_ * (1.0f - tw0) + fn * tw0 : fn;
+                f1[q] = use_old1 ? fo_ * (1.0f - tw1) + fn * tw1 : fn;
             } else {
-                fv[j] = j == 0 ? w_k : fv[0];
+                f0[q] = q == 0 ? w_k : f0[0];
+                f1[q] = q == 0 ? w_k : f1[0];
             }
         }
-        float c00 = fv[0] * (1.0f - wx) + fv[1] * wx;
-        float c01 = fv[4] * (1.0f - wx) + fv[5] * wx;
-        float c10 = fv[2] * (1.0f - wx) + fv[3] * wx;
-        float c11 = fv[6] * (1.0f - wx) + fv[7] * wx;
-        float c0 = c00 * (1.0f - wy) + c10 * wy;
-        float c1 = c01 * (1.0f - wy) + c11 * wy;
-        const float r0 = c0 * (1.0f - wz) + c1 * wz;
-        const float eq = a.thermal ? geq(r[0], r[1], r[2], r[3], w_k, cx, cy, cz)
-                                   : feq_auto(r[0], r[1], r[2], r[3], w_k, cx, cy, cz, a.comp);
-        const float neq = r0 - eq;
-        Codec<ST>::store(fo, fidx(gb, k, c), eq + neq * scale, w_k);
+        float rr0, rr1;
+        {
+            float c00 = f0[0] * (1.0f - wx) + f0[1] * wx;
+            float c01 = f0[4] * (1.0f - wx) + f0[5] * wx;
+            float c10 = f0[2] * (1.0f - wx) + f0[3] * wx;
+            float c11 = f0[6] * (1.0f - wx) + f0[7] * wx;
+            float c0 = c00 * (1.0f - wy) + c10 * wy;
+            float c1 = c01 * (1.0f - wy) + c11 * wy;
+            rr0 = c0 * (1.0f - wz) + c1 * wz;
+        }
+        {
+            float c00 = f1[0] * (1.0f - wx) + f1[1] * wx;
+            float c01 = f1[4] * (1.0f - wx) + f1[5] * wx;
+            float c10 = f1[2] * (1.0f - wx) + f1[3] * wx;
+            float c11 = f1[6] * (1.0f - wx) + f1[7] * wx;
+            float c0 = c00 * (1.0f - wy) + c10 * wy;
+            float c1 = c01 * (1.0f - wy) + c11 * wy;
+            rr1 = c0 * (1.0f - wz) + c1 * wz;
+        }
+        const float eq0 = a.thermal ? geq(r0[0], r0[1], r0[2], r0[3], w_k, cx, cy, cz)
+                                    : feq_auto(r0[0], r0[1], r0[2], r0[3], w_k, cx, cy, cz, a.comp);
+        const float eq1 = a.thermal ? geq(r1[0], r1[1], r1[2], r1[3], w_k, cx, cy, cz)
+                                    : feq_auto(r1[0], r1[1], r1[2], r1[3], w_k, cx, cy, cz, a.comp);
+        const float neq0 = rr0 - eq0, neq1 = rr1 - eq1;
+        Codec<ST>::store(fo, fidx(gb, k, c), eq0 + neq0 * scale, w_k);
+        Codec<ST>::store(fo2, fidx(gb, k, c), eq1 + neq1 * scale, w_k);
     }
 }
 

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "lb_geom.h"
+#include "lb_sync.h"
 #include "ludwig_hip.h"
 
 namespace {
@@ -125,6 +126,7 @@ int lb_bouzidi_qmap_gpu(int device, int64_t nb, const int32_t* coords, const int
                         uint16_t** q_out, int32_t** tri_out, int64_t* n_out) {
     if (!coords || !grid || !tris || !offset || !cells_out || !q_out || !tri_out || !n_out || nb < 0 || ntri < 0)
         return LB_EINVAL;
+    lbsync::SetupGuard guard;
     *cells_out = nullptr;
     *q_out = nullptr;
     *tri_out = nullptr;
