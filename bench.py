@@ -58,6 +58,8 @@ def main() -> int:
     ap.add_argument("--storage", default="fp32", choices=sorted(B_ALG_WMLES),
                     help="population storage (fp16/bf16: offsets f - w, FP32 arithmetic)")
     ap.add_argument("--cpu-steps", type=int, default=1, help="coarse steps timed for the CPU baseline (0: skip)")
+    ap.add_argument("--probe-gib", type=float, default=2.0,
+                    help="buffer size of the HBM bandwidth probe (0: skip)")
     ap.add_argument("--serving-requests", type=int, default=8,
                     help="requests in the serving comparison, FP16 batched vs plain FP32 (0: skip)")
     args = ap.parse_args()
@@ -151,6 +153,21 @@ def main() -> int:
     log(f"timed {args.steps} coarse steps in {elapsed:.3f}s: MLUPS_true {m_true:.0f}, MLUPS_ref {m_ref:.0f}; "
         f"finest collide {avg_ms:.3f} ms/launch = {achieved:.0f} GB/s; Cd {forces['Cd']:.4f}")
 
+    # practical HBM ceiling of plain streaming kernels on this device
+    probe = None
+    if args.probe_gib > 0:
+        import ctypes as C
+        import numpy as np
+        from openludwig_amd import _native as N
+        pb = np.zeros(5)
+        rc = N.hip_lib().lb_bandwidth_probe(0, int(args.probe_gib * (1 << 30)), 10, N.ptr(pb, C.c_double))
+        if rc == 0:
+            probe = {"copy": round(pb[0], 1), "read": round(pb[1], 1), "write": round(pb[2], 1),
+                     "copy_nontemporal": round(pb[3], 1), "copy_nontemporal_x4": round(pb[4], 1),
+                     "unit": "GB/s", "buffer_gib": args.probe_gib}
+            log(f"bandwidth probe: copy {pb[0]:.0f}, read {pb[1]:.0f}, write {pb[2]:.0f}, "
+                f"copy NT {pb[3]:.0f}, copy NT x4 {pb[4]:.0f} GB/s")
+
     cpu = None
     if args.cpu_steps > 0:
         sys.path.insert(0, str(ROOT / "oracle"))
@@ -201,7 +218,9 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "collide_kernel (finest level)", "bytes_per_launch": bytes_launch,
-                     "avg_launch_ms": round(avg_ms, 4)},
+                     "avg_launch_ms": round(avg_ms, 4),
+                     "traffic_GBps": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "probe": probe},
         "cpu_baseline": cpu,
         "serving": serving,
     }

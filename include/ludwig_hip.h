@@ -273,8 +273,10 @@ const char* lb_setup_gpu_last_error(void);
 
 /* HBM bandwidth probe (no counterpart in the reference): streams `bytes`
  * with 16-byte accesses; gbps_out[0] copy (read + write counted),
- * [1] read-only, [2] write-only, each averaged over `reps` launches.
- * The practical ceiling reported beside the nominal 8 TB/s.               */
+ * [1] read-only, [2] write-only, [3] copy with non-temporal loads and
+ * stores, [4] the same with four loads in flight per work-item (five
+ * doubles), each averaged over `reps` launches.  The practical ceiling
+ * reported beside the nominal 8 TB/s.                                       */
 int lb_bandwidth_probe(int device, int64_t bytes, int reps, double* gbps_out);
 
 #ifdef __cplusplus

@@ -94,6 +94,26 @@ class CaseConfig:
     # diagnostics / gpu
     diag_freq: int = 500
     gpu_async_depth: int = 8
+    # case folder and outputs (read by the case runner and the result writers)
+    case_dir: str = ""
+    output_dir: str = "output"
+    output_density: bool = True
+    output_velocity: bool = True
+    output_velocity_magnitude: bool = True
+    output_vorticity: bool = True
+    output_obstacle: bool = True
+    output_level: bool = True
+    output_bouzidi: bool = True
+    # read and kept for the summaries; they do not change the time step
+    tau_safety_factor: float = f32(1.0)
+    wall_model_type: str = "equilibrium"
+    wall_model_y_plus_target: float = 30.0
+    force_output_freq: int = 0
+    net_force_export: bool = False
+    stability_check: bool = True
+    print_tau_warning: bool = True
+    use_streams: bool = True
+    prefetch_neighbors: bool = True
 
     @property
     def ddf_enabled(self) -> bool:
@@ -174,11 +194,22 @@ def load_case_configuration(config_path: str | Path) -> CaseConfig:
     c.steps = int(_get(cfg, "basic", "simulation", "steps", default=0))
     c.ramp_steps = int(_get(cfg, "basic", "simulation", "ramp_steps", required=True))
     c.output_freq = int(_get(cfg, "basic", "simulation", "output_freq", required=True))
+    c.case_dir = str(case_dir)
+    c.output_dir = str(_get(cfg, "basic", "simulation", "output_dir", required=True))
+    of = ("basic", "simulation", "output_fields")
+    c.output_density = bool(_get(cfg, *of, "density", default=True))
+    c.output_velocity = bool(_get(cfg, *of, "velocity", default=True))
+    c.output_velocity_magnitude = bool(_get(cfg, *of, "velocity_magnitude", default=True))
+    c.output_vorticity = bool(_get(cfg, *of, "vorticity", default=True))
+    c.output_obstacle = bool(_get(cfg, *of, "obstacle", default=True))
+    c.output_level = bool(_get(cfg, *of, "level", default=True))
+    c.output_bouzidi = bool(_get(cfg, *of, "bouzidi", default=True))
     num = ("advanced", "numerics")
     c.collision_operator = str(_get(cfg, *num, "collision_operator", default="cumulant"))
     c.u_lattice = f32(_get(cfg, *num, "u_lattice", default=0.08))
     c.c_wale = f32(_get(cfg, *num, "c_wale", default=0.20))
     c.tau_min = f32(_get(cfg, *num, "tau_min", default=0.501))
+    c.tau_safety_factor = f32(_get(cfg, *num, "tau_safety_factor", default=1.0))
     c.inlet_turbulence_intensity = f32(_get(cfg, *num, "inlet_turbulence_intensity", default=0.01))
     c.nu_sgs_background = f32(_get(cfg, *num, "nu_sgs_background", default=0.0))
     c.sponge_blend_distributions = bool(_get(cfg, *num, "sponge_blend_distributions", default=False))
@@ -203,6 +234,8 @@ def load_case_configuration(config_path: str | Path) -> CaseConfig:
     c.max_levels = int(_get(cfg, *hr, "max_levels", default=12))
     c.min_coarse_blocks = int(_get(cfg, *hr, "min_coarse_blocks", default=4))
     c.wall_model_enabled = bool(_get(cfg, *hr, "wall_model", "enabled", default=False))
+    c.wall_model_type = str(_get(cfg, *hr, "wall_model", "type", default="equilibrium"))
+    c.wall_model_y_plus_target = float(_get(cfg, *hr, "wall_model", "y_plus_target", default=30.0))
     dm = ("advanced", "domain")
     c.domain_upstream = float(_get(cfg, *dm, "upstream", default=0.75))
     c.domain_downstream = float(_get(cfg, *dm, "downstream", default=1.5))
@@ -225,6 +258,14 @@ def load_case_configuration(config_path: str | Path) -> CaseConfig:
     c.forces_enabled = bool(_get(cfg, *fo, "enabled", default=True))
     mc = _get(cfg, *fo, "moment_center", default=[0.25, 0.0, 0.0])
     c.moment_center = tuple(float(v) for v in mc)
-    c.diag_freq = int(_get(cfg, "advanced", "diagnostics", "freq", default=500))
-    c.gpu_async_depth = int(_get(cfg, "advanced", "gpu", "async_depth", default=8))
+    c.force_output_freq = int(_get(cfg, *fo, "output_freq", default=0))
+    c.net_force_export = bool(_get(cfg, *fo, "net_force_export", default=False))
+    dg = ("advanced", "diagnostics")
+    c.diag_freq = int(_get(cfg, *dg, "freq", default=500))
+    c.stability_check = bool(_get(cfg, *dg, "stability_check", default=True))
+    c.print_tau_warning = bool(_get(cfg, *dg, "print_tau_warning", default=True))
+    gp = ("advanced", "gpu")
+    c.gpu_async_depth = int(_get(cfg, *gp, "async_depth", default=8))
+    c.use_streams = bool(_get(cfg, *gp, "use_streams", default=True))
+    c.prefetch_neighbors = bool(_get(cfg, *gp, "prefetch_neighbors", default=True))
     return c.validate()
