@@ -179,6 +179,15 @@ int lb_mesh_upload(lb_ctx* ctx, int64_t n_tri, const float* cx, const float* cy,
 int lb_forces(lb_ctx* ctx, const lb_force_params* fp, double* out, float* p_map, float* sx_map, float* sy_map,
               float* sz_map);
 
+/* The per-triangle surface maps (pressure and wall shear, Float32) of one
+ * side of the surface on the finest level: side = +1 is map_surface_stresses!
+ * (probe along the outward normal, surface.jl:350-507, 766-826; the maps
+ * lb_forces returns), side = -1 is map_minus_side_stresses! (surface.jl:
+ * 511-662, 923-954), used by compute_net_aerodynamics! for open surfaces.  The four
+ * arrays have n_tri floats each.                                           */
+int lb_surface_maps(lb_ctx* ctx, const lb_force_params* fp, int32_t side, float* p_map, float* sx_map,
+                    float* sy_map, float* sz_map);
+
 /* Field transfer in the reference layout (io_vtk.jl:46-51).               */
 int lb_download_level(lb_ctx* ctx, int level_id, int field, float* dst);
 int lb_upload_level(lb_ctx* ctx, int level_id, int field, const float* src);

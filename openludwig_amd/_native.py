@@ -76,6 +76,7 @@ HIP_EXPORTS = {
     "lb_synchronize": (C.c_int, [C.c_void_p]),
     "lb_mesh_upload": (C.c_int, [C.c_void_p, C.c_int64, f32p, f32p, f32p, f32p, f32p, f32p, f32p]),
     "lb_forces": (C.c_int, [C.c_void_p, C.POINTER(LbForceParams), f64p, f32p, f32p, f32p, f32p]),
+    "lb_surface_maps": (C.c_int, [C.c_void_p, C.POINTER(LbForceParams), C.c_int32, f32p, f32p, f32p, f32p]),
     "lb_download_level": (C.c_int, [C.c_void_p, C.c_int, C.c_int, f32p]),
     "lb_upload_level": (C.c_int, [C.c_void_p, C.c_int, C.c_int, f32p]),
     "lb_stats": (C.c_int, [C.c_void_p, C.c_int, f64p]),

@@ -1007,13 +1007,11 @@ int lb_mesh_upload(lb_ctx* c, int64_t n, const float* cx, const float* cy, const
     return LB_OK;
 }
 
-int lb_forces(lb_ctx* c, const lb_force_params* fp, double* out, float* p_map, float* sx_map, float* sy_map,
-              float* sz_map) {
-    lbsync::SetupGuard guard;
-    RC(check_ctx(c));
-    if (!fp || !out) return fail(c, LB_EINVAL, "null argument");
-    if (c->n_tri <= 0) return fail(c, LB_ESTATE, "no mesh uploaded");
-    HIPCHK(c, hipSetDevice(c->device));
+}  // extern "C"
+
+namespace {
+// The force kernel's arguments on the finest level for one side of the surface.
+lbk::ForceArgs force_args(lb_ctx* c, const lb_force_params* fp, float side) {
     const Level& L = c->levels.back();
     lbk::ForceArgs a;
     std::memset(&a, 0, sizeof a);
@@ -1047,6 +1045,21 @@ int lb_forces(lb_ctx* c, const lb_force_params* fp, double* out, float* p_map, f
     a.sy_map = c->maps[2];
     a.sz_map = c->maps[3];
     a.partials = c->fpart;
+    a.side = side;
+    return a;
+}
+}  // namespace
+
+extern "C" {
+
+int lb_forces(lb_ctx* c, const lb_force_params* fp, double* out, float* p_map, float* sx_map, float* sy_map,
+              float* sz_map) {
+    lbsync::SetupGuard guard;
+    RC(check_ctx(c));
+    if (!fp || !out) return fail(c, LB_EINVAL, "null argument");
+    if (c->n_tri <= 0) return fail(c, LB_ESTATE, "no mesh uploaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    const lbk::ForceArgs a = force_args(c, fp, 1.0f);
     const int groups = (int)((c->n_tri + 255) / 256);
     HIPCHK(c, c->k->forces(a, groups, nullptr, c->stream));
     std::vector<double> part((size_t)groups * 9);
@@ -1084,6 +1097,24 @@ int lb_forces(lb_ctx* c, const lb_force_params* fp, double* out, float* p_map, f
         out[LB_F_CMY] = My / M_ref;
         out[LB_F_CMZ] = Mz / M_ref;
     }
+    return LB_OK;
+}
+
+int lb_surface_maps(lb_ctx* c, const lb_force_params* fp, int32_t side, float* p_map, float* sx_map, float* sy_map,
+                    float* sz_map) {
+    lbsync::SetupGuard guard;
+    RC(check_ctx(c));
+    if (!fp || !p_map || !sx_map || !sy_map || !sz_map) return fail(c, LB_EINVAL, "null argument");
+    if (side != 1 && side != -1) return fail(c, LB_EINVAL, "side must be +1 or -1");
+    if (c->n_tri <= 0) return fail(c, LB_ESTATE, "no mesh uploaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    const lbk::ForceArgs a = force_args(c, fp, (float)side);
+    const int groups = (int)((c->n_tri + 255) / 256);
+    HIPCHK(c, c->k->forces(a, groups, nullptr, c->stream));
+    float* dst[4] = {p_map, sx_map, sy_map, sz_map};
+    for (int m = 0; m < 4; m++)
+        HIPCHK(c, hipMemcpyAsync(dst[m], c->maps[m], (size_t)c->n_tri * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return LB_OK;
 }
 

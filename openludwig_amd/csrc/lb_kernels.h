@@ -127,6 +127,7 @@ struct ForceArgs {
     float* sy_map;
     float* sz_map;
     double* partials;  // [n_groups][9]
+    float side;        // +1: probe along the outward normal; -1: the minus side
 };
 
 }  // namespace lbk

@@ -1448,13 +1448,18 @@ __device__ bool trilinear_sample(const ForceArgs& a, float px, float py, float p
 
 // map_stresses_kernel! + integrate_forces_kernel! per triangle; the nine sums
 // are reduced in FP64 in a fixed tree order per work-group (deterministic).
+// With side = -1 it is map_minus_side_only_kernel! (surface.jl:511-662):
+// that kernel is this one with the normal negated in the probe, the
+// fallback's side test, the wall distance and the stress (each a sign flip,
+// exact), so the maps are computed here with n -> -n; the integrated sums
+// are then not used.
 __global__ void __launch_bounds__(256) forces_kernel(const ForceArgs a) {
     __shared__ double red[9][256];
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float d9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (i < a.n_tri) {
         float tx = a.cx[i] + a.off_x, ty = a.cy[i] + a.off_y, tz = a.cz[i] + a.off_z;
-        float n_x = a.nx[i], n_y = a.ny[i], n_z = a.nz[i];
+        float n_x = a.side * a.nx[i], n_y = a.side * a.ny[i], n_z = a.side * a.nz[i];
         float brho = 1.0f, bux = 0.0f, buy = 0.0f, buz = 0.0f, bwd = 1.0f, bq = 0.0f;
         bool found = false;
         for (int pi = 1; pi <= a.search_radius; pi++) {

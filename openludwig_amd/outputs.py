@@ -5,11 +5,14 @@ forces.csv        src/forces/io.jl:5-22
 surface_NNNNNN.lbmp   export_surface_pressure_bin (src/forces/io.jl:230-335)
 surface_NNNNNN.vtu    save_surface_vtk (src/forces/io.jl:75-137)
 flow_NNNNNN.vtu       export_merged_mesh_sync (src/io/io_vtk.jl:6-129)
+net_forces_NNNNNN.csv / .vtp   export_net_triangle_forces_csv / _vtk
+                      (src/forces/io.jl:139-228, 337-444)
 
-The CSV rows use the reference's printf formats, so files written by either
-program diff line by line.  The .lbmp writer produces the reference's byte
-layout (little-endian header, name table, contiguous Float32/Int32 arrays)
-with the same Float32 rounding of every value.  The VTU files are VTK XML
+The CSV rows and the net-force VTP (an ASCII file) use the reference's
+printf formats, so files written by either program diff line by line.  The
+.lbmp writer produces the reference's byte layout (little-endian header,
+name table, contiguous Float32/Int32 arrays) with the same Float32 rounding
+of every value.  The VTU files are VTK XML
 UnstructuredGrid files holding the reference's arrays under its names; they
 are written with inline base64 binary data (WriteVTK's compressed form is
 one of several encodings VTK readers accept), and `read_vtu` reads them back.
@@ -328,4 +331,84 @@ def export_flow_vtu(path: str | Path, levels: list, cfg, compress: bool = True) 
     out = Path(str(path) + ".vtu")
     write_vtu(out, points, conn.ravel(), 8 * np.arange(1, n * ncl + 1), np.full(n * ncl, VTK_VOXEL), data,
               compress)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Net per-triangle forces
+
+def export_net_triangle_forces_csv(path: str | Path, net: tuple, centers: np.ndarray, offset=(0.0, 0.0, 0.0),
+                                   alpha_deg: float = 0.0, beta_deg: float = 0.0, object_names=None,
+                                   unrotated: bool = True) -> None:
+    """export_net_triangle_forces_csv (src/forces/io.jl:139-228).
+
+    With `unrotated` (what main.jl passes), `centers` are the unrotated
+    triangle centres, no offset is added, and the Float32 force vectors are
+    turned back into the STL frame with R^T = (Rz(beta) Ry(alpha))^T, its
+    entries rounded to Float32 and applied in Float32.  Otherwise `centers`
+    are the solver mesh's Float32 centres and `offset` is added."""
+    f32 = np.float32
+    fx, fy, fz = (np.asarray(a, f32) for a in net)
+    c = np.asarray(centers, np.float64).astype(f32)
+    if unrotated:
+        off = (f32(0.0), f32(0.0), f32(0.0))
+    else:
+        off = tuple(f32(v) for v in offset)
+    if unrotated and (alpha_deg != 0.0 or beta_deg != 0.0):
+        a, b = np.deg2rad(alpha_deg), np.deg2rad(beta_deg)
+        ca, sa, cb, sb = np.cos(a), np.sin(a), np.cos(b), np.sin(b)
+        r = [[f32(ca * cb), f32(-ca * sb), f32(sa)],
+             [f32(sb), f32(cb), f32(0.0)],
+             [f32(-sa * cb), f32(sa * sb), f32(ca)]]
+        fx, fy, fz = (r[i][0] * fx + r[i][1] * fy + r[i][2] * fz for i in range(3))
+    names = object_names if object_names is not None else ["default"] * len(fx)
+    lines = ["triangle_id,object_name,cx,cy,cz,Fx,Fy,Fz"]
+    cx, cy, cz = c[:, 0] + off[0], c[:, 1] + off[1], c[:, 2] + off[2]
+    for i in range(len(fx)):
+        lines.append("%d,%s,%.6e,%.6e,%.6e,%.6e,%.6e,%.6e" % (i + 1, names[i], cx[i], cy[i], cz[i],
+                                                             fx[i], fy[i], fz[i]))
+    Path(path).parent.mkdir(parents=True, exist_ok=True)
+    Path(path).write_text("\n".join(lines) + "\n")
+
+
+def export_net_triangle_forces_vtk(path: str | Path, net: tuple, centers: np.ndarray, normals: np.ndarray,
+                                   areas: np.ndarray, offset=(0.0, 0.0, 0.0)) -> Path:
+    """export_net_triangle_forces_vtk (src/forces/io.jl:337-444): an ASCII
+    VTK PolyData of one vertex per triangle centre (Float32 centre + Float64
+    offset) with NetForce, NetForceMagnitude, Normal and Area."""
+    f32 = np.float32
+    fx, fy, fz = (np.asarray(a, f32) for a in net)
+    c = np.asarray(centers, np.float64).astype(f32).astype(np.float64)
+    nrm = np.asarray(normals, np.float64).astype(f32)
+    ar = np.asarray(areas, np.float64).astype(f32)
+    n = len(fx)
+    mag = np.sqrt(fx * fx + fy * fy + fz * fz)
+    o = [float(v) for v in offset]
+    L = ['<?xml version="1.0"?>',
+         '<VTKFile type="PolyData" version="1.0" byte_order="LittleEndian">',
+         "  <PolyData>",
+         f'    <Piece NumberOfPoints="{n}" NumberOfVerts="{n}" NumberOfLines="0" NumberOfStrips="0" '
+         f'NumberOfPolys="0">',
+         "      <Points>",
+         '        <DataArray type="Float64" NumberOfComponents="3" format="ascii">']
+    L += ["          %.8e %.8e %.8e" % (c[i, 0] + o[0], c[i, 1] + o[1], c[i, 2] + o[2]) for i in range(n)]
+    L += ["        </DataArray>", "      </Points>", "      <Verts>",
+          '        <DataArray type="Int64" Name="connectivity" format="ascii">']
+    L += ["          %d" % i for i in range(n)]
+    L += ["        </DataArray>", '        <DataArray type="Int64" Name="offsets" format="ascii">']
+    L += ["          %d" % i for i in range(1, n + 1)]
+    L += ["        </DataArray>", "      </Verts>", '      <PointData Vectors="NetForce">',
+          '        <DataArray type="Float32" Name="NetForce" NumberOfComponents="3" format="ascii">']
+    L += ["          %.6e %.6e %.6e" % (fx[i], fy[i], fz[i]) for i in range(n)]
+    L += ["        </DataArray>", '        <DataArray type="Float32" Name="NetForceMagnitude" format="ascii">']
+    L += ["          %.6e" % mag[i] for i in range(n)]
+    L += ["        </DataArray>",
+          '        <DataArray type="Float32" Name="Normal" NumberOfComponents="3" format="ascii">']
+    L += ["          %.6e %.6e %.6e" % (nrm[i, 0], nrm[i, 1], nrm[i, 2]) for i in range(n)]
+    L += ["        </DataArray>", '        <DataArray type="Float32" Name="Area" format="ascii">']
+    L += ["          %.6e" % ar[i] for i in range(n)]
+    L += ["        </DataArray>", "      </PointData>", "    </Piece>", "  </PolyData>", "</VTKFile>"]
+    out = Path(str(path) + ".vtp")
+    out.parent.mkdir(parents=True, exist_ok=True)
+    out.write_text("\n".join(L) + "\n")
     return out

@@ -20,8 +20,8 @@ Deliberate differences: only the files this runner writes are removed from
 an existing output folder (the reference empties the folder); the flow VTU
 holds each level's current velocity (the reference picks the velocity pair
 half by the coarse step's parity, which on refined levels can be the previous
-substep's); the Markdown analysis summaries, the VRAM report and the
-two-sided net forces are not written.
+substep's); the Markdown analysis summaries and the VRAM report are not
+written.
 """
 from __future__ import annotations
 
@@ -40,7 +40,8 @@ from .geometry import load_mesh, merge_meshes
 from .solver import ForceAccumulator, Solver, build_case, ramp_velocity
 
 STORAGE = {"fp32": N.LB_STORAGE_FP32, "fp16": N.LB_STORAGE_FP16, "bf16": N.LB_STORAGE_BF16}
-_OWN_FILES = re.compile(r"^(convergence\.csv|forces\.csv|(surface|flow)_\d{6}(\.vtu|\.lbmp))$")
+_OWN_FILES = re.compile(r"^(convergence\.csv|forces\.csv|(surface|flow)_\d{6}(\.vtu|\.lbmp)|"
+                        r"net_forces_\d{6}\.(csv|vtp))$")
 
 
 def load_case_meshes(cfg: CaseConfig):
@@ -203,6 +204,14 @@ def _outputs(out, step, solver, cfg, acc, extremes, unrotated, object_names, wri
         for k in extremes:
             extremes[k] = [min(extremes[k][0], fd[k]), max(extremes[k][1], fd[k])]
     _surface_files(out, step, solver, fd, unrotated, object_names)
+    if cfg.net_force_export:
+        # compute_net_aerodynamics! + the two net-force exports (main.jl:289-307)
+        net = solver.compute_net_aerodynamics()
+        m = solver.mesh
+        O.export_net_triangle_forces_csv(out / f"net_forces_{step:06d}.csv", net, unrotated.centers,
+                                         alpha_deg=cfg.alpha_deg, beta_deg=cfg.beta_deg, object_names=object_names)
+        O.export_net_triangle_forces_vtk(out / f"net_forces_{step:06d}", net, m.centers, m.normals, m.areas,
+                                         offset=solver.params.mesh_offset)
     return fd
 
 

@@ -198,7 +198,7 @@ class Solver:
     def synchronize(self) -> None:
         self._check(self.lib.lb_synchronize(self._ctx), "lb_synchronize")
 
-    def compute_aerodynamics(self, maps: bool = False) -> dict:
+    def _force_params(self) -> N.LbForceParams:
         p = self.params
         fp = N.LbForceParams()
         for i in range(3):
@@ -214,6 +214,11 @@ class Solver:
         fp.nu_sgs_background = self.cfg.nu_sgs_background
         fp.search_radius = 5
         fp.symmetric = int(self.cfg.symmetric_analysis)
+        return fp
+
+    def compute_aerodynamics(self, maps: bool = False) -> dict:
+        """compute_aerodynamics! (surface.jl:913-919) on the finest level."""
+        fp = self._force_params()
         out = np.zeros(N.LB_F_N)
         mp = [np.zeros(self.mesh.n, np.float32) for _ in range(4)] if maps else [None] * 4
         self._check(self.lib.lb_forces(self._ctx, C.byref(fp), N.ptr(out, C.c_double),
@@ -222,6 +227,32 @@ class Solver:
         if maps:
             res["maps"] = mp
         return res
+
+    def surface_maps(self, side: int = 1) -> list:
+        """Per-triangle [p, tau_x, tau_y, tau_z] (Float32) of the plus (+1) or
+        minus (-1) side of the surface (lb_surface_maps)."""
+        mp = [np.zeros(self.mesh.n, np.float32) for _ in range(4)]
+        self._check(self.lib.lb_surface_maps(self._ctx, C.byref(self._force_params()), int(side),
+                                             *[N.ptr(m, C.c_float) for m in mp]), "lb_surface_maps")
+        return mp
+
+    def compute_net_aerodynamics(self, closed_body: bool = True) -> tuple:
+        """compute_net_aerodynamics! (surface.jl:980-1035): per-triangle net
+        force F = (-p_net n + tau_net) A with p_net = p_plus - p_minus, in
+        Float32 as compute_net_triangle_forces_kernel! (surface.jl:665-700).
+        A closed body (the reference's default) has no fluid inside, so the
+        minus side is zero; an open surface probes both sides."""
+        plus = self.surface_maps(1)
+        minus = [np.zeros_like(m) for m in plus] if closed_body else self.surface_maps(-1)
+        f32 = np.float32
+        nx, ny, nz = (self.mesh.normals[:, i].astype(f32) for i in range(3))
+        area = self.mesh.areas.astype(f32)
+        p_net = plus[0] - minus[0]
+        s_net = [plus[i] - minus[i] for i in (1, 2, 3)]
+        out = []
+        for n_c, s_c in zip((nx, ny, nz), s_net):
+            out.append(((-p_net) * n_c) * area + s_c * area)
+        return tuple(out)
 
     def download(self, level_id: int, field: str) -> np.ndarray:
         lev = self.grids[level_id - 1]

@@ -170,6 +170,40 @@ def test_forces_match_oracle(hip_device):
 
 
 @pytest.mark.gpu
+def test_surface_maps_both_sides_and_net_forces(hip_device):
+    """lb_surface_maps: the plus side equals lb_forces' maps; the minus side
+    (map_minus_side_only_kernel!) equals the oracle's stress mapping on the
+    mesh with negated normals, bit for bit (the reference's minus kernel is
+    the plus kernel with n -> -n).  Closed-body net forces summed in FP64
+    give lb_forces' total force to Float32 rounding of the per-triangle sums."""
+    import dataclasses
+
+    from oracle import oracle_forces
+    cfg, mesh = cases.tiny_sphere(8, 2)
+    grids, params, mesh = build_case(cfg, mesh)
+    s, o = run_pair(cfg, grids, params, mesh, steps=16, strict=True)
+    fg = s.compute_aerodynamics(maps=True)
+    plus = s.surface_maps(1)
+    minus = s.surface_maps(-1)
+    for i in range(4):
+        assert np.array_equal(plus[i], fg["maps"][i]), f"plus map {i}"
+    neg = dataclasses.replace(mesh, normals=-mesh.normals)
+    fo = oracle_forces(o, cfg, params, neg, grids[-1].dx)
+    for i in range(4):
+        assert np.array_equal(minus[i], fo["maps"][i]), f"minus map {i}"
+    net = s.compute_net_aerodynamics(closed_body=True)
+    for c, key in enumerate(("Fx", "Fy", "Fz")):
+        tot = float(np.sum(net[c].astype(np.float64)))
+        mag = float(np.sum(np.abs(net[c].astype(np.float64))))
+        assert abs(tot - fg[key]) <= 1e-6 * mag + 1e-12, f"{key}: {tot} vs {fg[key]}"
+    # open surface: the minus side (here fluid across the small sphere) enters the net force
+    open_net = s.compute_net_aerodynamics(closed_body=False)
+    assert all(np.all(np.isfinite(c)) for c in open_net)
+    hit = minus[0] != 0
+    assert np.any(hit) and not np.array_equal(open_net[0][hit], net[0][hit])
+
+
+@pytest.mark.gpu
 def test_uniform_pressure_closed_mesh_zero_force(hip_device):
     """Sum n_i A_i = 0 on a closed mesh: a uniform density (any value) and zero
     velocity give zero pressure force to rounding."""

@@ -49,6 +49,8 @@ advanced:
     height: 1.0
   diagnostics:
     freq: 16
+  forces:
+    net_force_export: true
 """
 
 
@@ -79,6 +81,12 @@ def test_runner_outputs_and_coefficients(hip_device, tmp_path):
         assert s["n_cells"] == 12 and b["pressure"].shape == (12,)
         assert np.array_equal(s["cell_data"]["Pressure"], b["pressure"])
         assert b["names"] == ["cube1m"]
+    # net per-triangle forces (closed body): their sum is the total force
+    _, nrows = O.read_csv(out / "net_forces_000048.csv")
+    assert len(nrows) == 12 and (out / "net_forces_000048.vtp").is_file()
+    fx = sum(float(r[5]) for r in nrows)
+    fx_total = float(frows[-1][3])
+    assert abs(fx - fx_total) <= 1e-4 * sum(abs(float(r[5])) for r in nrows) + 1e-9
     f = O.read_vtu(out / "flow_000048.vtu")
     assert {"Density", "Velocity", "VelocityMagnitude", "Obstacle", "Level"} <= set(f["cell_data"])
     assert np.all(np.isfinite(f["cell_data"]["Density"]))
