@@ -7,6 +7,8 @@ sphere_re1e3     D = 1 m icosphere (subdiv 4, 5120 triangles), Re 1e3, coarse
 cube_re1e5       1 m cube (12-facet STL refined with minimum facet 0.025 to
                  49,152 triangles), Re 1e5, cumulant + WALE + WMLES, 4 levels,
                  ~50 M active cells in 8^3 blocks, u_lat 0.05.
+sphere_re1e5     D = 1 m icosphere (subdiv 6, 81,920 triangles) at Re 1e5 with
+                 WMLES, the second body of the precision form.
 tiny_sphere      a 2-level sphere case small enough for the CPU oracle.
 naca_wing_thermal  extruded NACA 0012 (chord 1 m, span 2 m, 40,400
                  triangles) at alpha 2 deg, compressible form: thermal DDF +
@@ -56,6 +58,12 @@ def cube_re1e5(resolution: int = 96, num_levels: int = 4, facet: float = 0.025):
     if facet > 0:
         mesh = subdivide_mesh(mesh, facet)
     return cfg, mesh
+
+
+def sphere_re1e5(resolution: int = 96, num_levels: int = 4, subdiv: int = 6):
+    cfg, _ = sphere_re1e3(resolution, num_levels)
+    cfg = cfg.replace(flow_velocity=1.5, wall_model_enabled=True, ramp_steps=2000, steps=2500, diag_freq=200)
+    return cfg, mesh_from_triangles(icosphere(subdiv, 0.5))
 
 
 def tiny_sphere(resolution: int = 8, num_levels: int = 2, wall_model: bool = False, limiter: str = "positivity",

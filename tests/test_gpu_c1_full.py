@@ -1,0 +1,58 @@
+"""The plain CPU-runnable case at its full length on the GPU: sphere D = 1 m
+(icosphere subdiv 4), Re 1e3, coarse 64x32x32, 3 levels, 2000 coarse steps
+with the main loop's cadence (ramp 500, batches of 8, forces every 100
+steps), against the FP64 oracle's run of the same case, recorded in
+golden/c1_sphere_re1e3_fp64.json by golden/make_c1_fp64.py.
+
+Bars: the FP32 drop-in (fast build) keeps Cd within 0.5 % of the FP64
+oracle at every diagnostic step after the ramp and in the post-ramp average
+(the plain form's target); Cl, near zero for the sphere, within 0.5 % of Cd.
+The FP16 storage form is held to the same bars on the average.
+"""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import pytest
+
+from openludwig_amd import _native as N
+from openludwig_amd import cases
+from openludwig_amd.solver import Solver, build_case
+
+FIXTURE = Path(__file__).parent / "golden" / "c1_sphere_re1e3_fp64.json"
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not FIXTURE.is_file(), reason="FP64 reference run not recorded")
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_c1_full_run_vs_fp64_oracle(hip_device, storage):
+    ref = json.loads(FIXTURE.read_text())
+    cfg, mesh = cases.sphere_re1e3()
+    grids, params, mesh = build_case(cfg, mesh)
+    assert [g.n_cells for g in grids] == ref["cells_per_level"]
+    st = N.LB_STORAGE_FP32 if storage == "fp32" else N.LB_STORAGE_FP16
+    s = Solver(cfg, grids, params, mesh, storage=st)
+    try:
+        s.init_equilibrium()
+        run = s.run(ref["steps"])
+    finally:
+        s.close()
+    hist = {h["step"]: h for h in run["history"]}
+    rows = []
+    for h in ref["history"]:
+        g = hist[h["step"]]
+        rows.append((h["step"], g["Cd"], h["Cd"], g["Cl"], h["Cl"]))
+    print(f"\nC1 {storage}: step  Cd gpu / fp64  Cl gpu / fp64")
+    for r in rows:
+        print("  %5d  %.6f / %.6f  %+.6f / %+.6f" % r)
+    print(f"  post-ramp average: Cd {run['Cd_avg']:.6f} vs {ref['Cd_avg']:.6f}, "
+          f"Cl {run['Cl_avg']:+.6f} vs {ref['Cl_avg']:+.6f} ({run['n_samples']} samples)")
+    assert run["n_samples"] == ref["n_samples"]
+    assert abs(run["Cd_avg"] - ref["Cd_avg"]) <= 5e-3 * abs(ref["Cd_avg"])
+    assert abs(run["Cl_avg"] - ref["Cl_avg"]) <= 5e-3 * abs(ref["Cd_avg"])
+    if storage == "fp32":
+        for step, cd, cd64, cl, cl64 in rows:
+            if step > cfg.ramp_steps:
+                assert abs(cd - cd64) <= 5e-3 * abs(cd64), f"step {step}: Cd {cd} vs {cd64}"
+                assert abs(cl - cl64) <= 5e-3 * abs(cd64), f"step {step}: Cl {cl} vs {cl64}"
