@@ -277,6 +277,19 @@ int lb_engine_set_options(lb_engine* e, int32_t use_graph, int32_t schedule, int
 int lb_bouzidi_qmap_gpu(int device, int64_t nb, const int32_t* coords, const int32_t* grid, int gdx, int gdy,
                         int gdz, const double* tris, int64_t ntri, double dx, const double* offset,
                         int32_t** cells_out, uint16_t** q_out, int32_t** tri_out, int64_t* n_out);
+/* voxelize_blocks! (domain_generation.jl:74-112) on the GPU: sets obst[b*512
+ * + cell] = 1 for every cell whose 1.5 dx box a candidate triangle crosses
+ * (SAT test, FP64, no contraction); other entries are left as they are.
+ * Same arguments and the same result as lbs_voxelize of the setup library. */
+int lb_voxelize_gpu(int device, int64_t nb, const int32_t* coords, const int32_t* grid, int gdx, int gdy, int gdz,
+                    const double* tris, int64_t ntri, double dx, const double* offset, uint8_t* obst);
+/* compute_wall_distances! (domain_generation.jl:371-434) on the GPU: the
+ * distance (FP32 metres) from each fluid cell to its nearest solid
+ * neighbour among the 26, written into wall_dist for near-wall cells only;
+ * *count_out (may be NULL) = number of such cells.  Same result as
+ * lbs_wall_distances.                                                       */
+int lb_wall_distances_gpu(int device, int64_t nb, const int32_t* coords, const int32_t* grid, int gdx, int gdy,
+                          int gdz, const uint8_t* obst, double dx, float* wall_dist, int64_t* count_out);
 void lb_free(void* p);
 const char* lb_setup_gpu_last_error(void);
 

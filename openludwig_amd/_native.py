@@ -100,6 +100,10 @@ HIP_EXPORTS = {
     "lb_device_memory": (C.c_int, [C.c_int, i64p, i64p]),
     "lb_bouzidi_qmap_gpu": (C.c_int, [C.c_int, C.c_int64, i32p, i32p, C.c_int, C.c_int, C.c_int, f64p, C.c_int64,
                                       C.c_double, f64p, C.POINTER(i32p), C.POINTER(u16p), C.POINTER(i32p), i64p]),
+    "lb_voxelize_gpu": (C.c_int, [C.c_int, C.c_int64, i32p, i32p, C.c_int, C.c_int, C.c_int, f64p, C.c_int64,
+                                  C.c_double, f64p, u8p]),
+    "lb_wall_distances_gpu": (C.c_int, [C.c_int, C.c_int64, i32p, i32p, C.c_int, C.c_int, C.c_int, u8p, C.c_double,
+                                        f32p, i64p]),
     "lb_free": (None, [C.c_void_p]),
     "lb_setup_gpu_last_error": (C.c_char_p, []),
 }

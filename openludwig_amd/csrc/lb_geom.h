@@ -1,7 +1,7 @@
 // lb_geom.h -- FP64 geometry shared by the CPU setup library (lb_setup.cpp,
 // g++) and its GPU counterpart (lb_setup_gpu.hip, hipcc): vectors, the
-// Moller-Trumbore ray-triangle test, Float64 -> Float16 rounding and the
-// block -> candidate-triangle map.  Both compile it without FMA contraction,
+// Moller-Trumbore ray-triangle test, the triangle-box SAT test, Float64 ->
+// Float16 rounding and the block -> candidate-triangle map.  Both compile it without FMA contraction,
 // so the two produce the same bits.
 #pragma once
 #include <algorithm>
@@ -101,6 +101,32 @@ LB_HD inline bool ray_tri(V3 o, V3 d, V3 v1, V3 v2, V3 v3, double* t_out) {
         return true;
     }
     return false;
+}
+
+LB_HD inline double min3(double a, double b, double c) { return a < b ? (a < c ? a : c) : (b < c ? b : c); }
+LB_HD inline double max3(double a, double b, double c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
+
+// triangle_intersects_aabb (src/geometry/domain_generation.jl:10-32): the
+// separating-axis test of a triangle against the box of half size hb (grown
+// by 0.1 %) centred at c; 3 box axes, then the 9 edge cross products (the
+// triangle normal axis is not tested, as in the reference).
+LB_HD inline bool tri_box(V3 c, double hb, V3 v1, V3 v2, V3 v3) {
+    const double h = hb * 1.001;
+    const V3 t1 = sub(v1, c), t2 = sub(v2, c), t3 = sub(v3, c);
+    if (min3(t1.x, t2.x, t3.x) > h || max3(t1.x, t2.x, t3.x) < -h) return false;
+    if (min3(t1.y, t2.y, t3.y) > h || max3(t1.y, t2.y, t3.y) < -h) return false;
+    if (min3(t1.z, t2.z, t3.z) > h || max3(t1.z, t2.z, t3.z) < -h) return false;
+    const V3 f[3] = {sub(t2, t1), sub(t3, t2), sub(t1, t3)};
+    const V3 u[3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            const V3 axis = cross(u[i], f[j]);
+            if (dot(axis, axis) < 1e-10) continue;
+            const double p1 = dot(t1, axis), p2 = dot(t2, axis), p3 = dot(t3, axis);
+            const double r = h * std::fabs(axis.x) + h * std::fabs(axis.y) + h * std::fabs(axis.z);
+            if (min3(p1, p2, p3) > r || max3(p1, p2, p3) < -r) return false;
+        }
+    return true;
 }
 
 // Dense 1-based block pointer grid helper.

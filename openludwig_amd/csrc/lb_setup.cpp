@@ -33,26 +33,6 @@ namespace {
 
 using namespace lbgeom;
 
-// triangle_intersects_aabb: src/geometry/domain_generation.jl:10-32
-bool tri_box(V3 c, double hb, V3 v1, V3 v2, V3 v3) {
-    double h = hb * 1.001;
-    V3 t1 = sub(v1, c), t2 = sub(v2, c), t3 = sub(v3, c);
-    if (std::min({t1.x, t2.x, t3.x}) > h || std::max({t1.x, t2.x, t3.x}) < -h) return false;
-    if (std::min({t1.y, t2.y, t3.y}) > h || std::max({t1.y, t2.y, t3.y}) < -h) return false;
-    if (std::min({t1.z, t2.z, t3.z}) > h || std::max({t1.z, t2.z, t3.z}) < -h) return false;
-    V3 f[3] = {sub(t2, t1), sub(t3, t2), sub(t1, t3)};
-    V3 u[3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-            V3 axis = cross(u[i], f[j]);
-            if (dot(axis, axis) < 1e-10) continue;
-            double p1 = dot(t1, axis), p2 = dot(t2, axis), p3 = dot(t3, axis);
-            double r = h * std::fabs(axis.x) + h * std::fabs(axis.y) + h * std::fabs(axis.z);
-            if (std::min(p1, std::min(p2, p3)) > r || std::max(p1, std::max(p2, p3)) < -r) return false;
-        }
-    return true;
-}
-
 thread_local char g_err[512];
 
 }  // namespace

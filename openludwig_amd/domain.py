@@ -267,6 +267,11 @@ def should_use_bouzidi(level_id: int, num_levels: int, method: str, bouzidi_leve
     return method == "bouzidi" and level_id > num_levels - bouzidi_levels
 
 
+def _check_gpu(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {N.hip_lib().lb_setup_gpu_last_error().decode()}")
+
+
 def bouzidi_qmap(coords: np.ndarray, dense_c: np.ndarray, dims: tuple, tris: np.ndarray, dx: float, off: np.ndarray,
                  gpu_device: int | None = None) -> tuple:
     """compute_bouzidi_qmap_sparse (bouzidi_setup.jl:64-167): (cells (n,4),
@@ -304,7 +309,8 @@ def setup_multilevel_domain(cfg: CaseConfig, mesh: SolverMesh, params: DomainPar
     """setup_multilevel_domain(mesh, params) (domain.jl:1-246).  With
     topology_only, only the block sets are built and the block count of each
     level is returned (used to size serving requests).  With gpu_device, the
-    Bouzidi Q-map is computed on that HIP device (lb_bouzidi_qmap_gpu, the
+    voxelization, the wall distances and the Bouzidi Q-map run on that HIP
+    device (lb_voxelize_gpu, lb_wall_distances_gpu, lb_bouzidi_qmap_gpu: the
     same bits as the CPU restatement)."""
     lib = N.setup_lib()
     off = np.ascontiguousarray(params.mesh_offset, dtype=np.float64)
@@ -369,16 +375,27 @@ def setup_multilevel_domain(cfg: CaseConfig, mesh: SolverMesh, params: DomainPar
         wall = np.full((nb, 512), 100.0, dtype=np.float32)
         ci = N.ptr(coords, C.c_int32)
         gi = N.ptr(dense_c, C.c_int32)
-        N.check_setup(lib.lbs_voxelize(nb, ci, gi, dims[0], dims[1], dims[2], N.ptr(tris, C.c_double),
-                                       tris.shape[0], dx, N.ptr(off, C.c_double), N.ptr(obst, C.c_uint8)),
-                      "lbs_voxelize")
+        if gpu_device is None:
+            N.check_setup(lib.lbs_voxelize(nb, ci, gi, dims[0], dims[1], dims[2], N.ptr(tris, C.c_double),
+                                           tris.shape[0], dx, N.ptr(off, C.c_double), N.ptr(obst, C.c_uint8)),
+                          "lbs_voxelize")
+        else:
+            _check_gpu(N.hip_lib().lb_voxelize_gpu(gpu_device, nb, ci, gi, dims[0], dims[1], dims[2],
+                                                   N.ptr(tris, C.c_double), tris.shape[0], dx,
+                                                   N.ptr(off, C.c_double), N.ptr(obst, C.c_uint8)),
+                       "lb_voxelize_gpu")
         filled = lib.lbs_flood_fill(nb, ci, gi, dims[0], dims[1], dims[2], N.ptr(obst, C.c_uint8))
         N.check_setup(lib.lbs_sponge(nb, ci, dx, params.domain_size[0], params.domain_size[1],
                                      params.domain_size[2], float(cfg.sponge_thickness),
                                      int(cfg.symmetric_analysis), N.ptr(sponge, C.c_float)), "lbs_sponge")
         if cfg.wall_model_enabled:
-            lib.lbs_wall_distances(nb, ci, gi, dims[0], dims[1], dims[2], N.ptr(obst, C.c_uint8), dx,
-                                   N.ptr(wall, C.c_float))
+            if gpu_device is None:
+                lib.lbs_wall_distances(nb, ci, gi, dims[0], dims[1], dims[2], N.ptr(obst, C.c_uint8), dx,
+                                       N.ptr(wall, C.c_float))
+            else:
+                _check_gpu(N.hip_lib().lb_wall_distances_gpu(gpu_device, nb, ci, gi, dims[0], dims[1], dims[2],
+                                                             N.ptr(obst, C.c_uint8), dx, N.ptr(wall, C.c_float),
+                                                             None), "lb_wall_distances_gpu")
         lev = BlockLevel(level_id=lvl, dx=f32(dx), tau=params.tau_levels[lvl - 1],
                          tau_g=params.tau_g_levels[lvl - 1] if params.tau_g_levels else 0.0,
                          coords=coords, neighbor_table=nbt,
