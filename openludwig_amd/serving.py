@@ -8,10 +8,10 @@ a fixed amount of simulated time.  Two ways to serve a stream of requests:
 run_sequential   the reference's way (run_all_cases, main.jl:425-469): one
                  case at a time -- set up, run through execute_timestep_batch!
                  in batches of gpu_async_depth, compute forces, free.
-run_batched      continuous batching: set-up (CPU tables with the Bouzidi
-                 Q-map on the GPU, then device upload) runs on worker threads
-                 as requests arrive; a case is
-                 admitted when its exact device footprint (lb_footprint) fits
+run_batched      continuous batching: set-up (block topology on the CPU;
+                 voxelization, wall distances and Bouzidi Q-map on the GPU;
+                 then device upload) runs on worker threads as requests
+                 arrive; a case is admitted when its exact device footprint (lb_footprint) fits
                  the HBM budget; admitted cases are attached to one Engine and
                  advance together in batched launches replayed as a hipGraph;
                  finished cases are detached, their forces computed and their
@@ -251,8 +251,8 @@ def run_batched(requests: list, storage: int = N.LB_STORAGE_FP16, strict_fp: boo
     """Continuous batching over one Engine (see the module docstring).
     Launch options: `options` if given, else autotuned once per (storage,
     deepest level count) on the first cases admitted, and reused.  With
-    gpu_setup, the Bouzidi Q-map of each case is computed on the GPU (the same
-    bits as the CPU restatement the plain form uses)."""
+    gpu_setup, the voxelization, wall distances and Bouzidi Q-map of each case
+    run on the GPU (the same bits as the CPU restatement the plain form uses)."""
     if hbm_budget is None:
         free, _total = device_memory()
         hbm_budget = free - reserve

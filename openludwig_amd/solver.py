@@ -328,8 +328,8 @@ class Solver:
 
 def build_case(cfg: CaseConfig, mesh: SolverMesh, verbose: bool = False, gpu_device: int | None = None):
     """setup_multilevel_domain(stl_paths) (domain.jl:248-295) from a mesh
-    already loaded: rotate, scale the domain, build every level (the Bouzidi
-    Q-map on `gpu_device` if given)."""
+    already loaded: rotate, scale the domain, build every level (the
+    voxelization, wall distances and Bouzidi Q-map on `gpu_device` if given)."""
     mesh = rotate_mesh(mesh, cfg.alpha_deg, cfg.beta_deg)
     params = compute_domain_from_mesh(cfg, mesh.min_bounds, mesh.max_bounds)
     grids = setup_multilevel_domain(cfg, mesh, params, verbose=verbose, gpu_device=gpu_device)
