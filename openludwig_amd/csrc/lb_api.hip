@@ -42,7 +42,7 @@ struct Kern {
     hipError_t (*collide)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
                           int, int, int, hipStream_t);
     hipError_t (*thermal)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
-                          int, hipStream_t);
+                          int, int, hipStream_t);
     hipError_t (*ghost_fill)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
                              hipStream_t);
     hipError_t (*bouzidi)(const lbk::LevelArgs*, const lbk::CaseState*, const lbk::Work*, int32_t, int64_t, int, int,
@@ -151,9 +151,10 @@ struct EngineCase {
 
 // Work lists of one depth of the engine's recursion (concatenated over cases).
 struct DepthWork {
-    lbk::Work *col = nullptr, *colb = nullptr, *gh = nullptr, *lk = nullptr, *tcol = nullptr, *tgh = nullptr,
+    lbk::Work *col = nullptr, *colb = nullptr, *gh = nullptr, *lk = nullptr, *tcol = nullptr, *tcolb = nullptr,
+              *tgh = nullptr,
               *tlk = nullptr;
-    int32_t ncol = 0, ncolb = 0, ngh = 0, nlk = 0, ntcol = 0, ntgh = 0, ntlk = 0;
+    int32_t ncol = 0, ncolb = 0, ngh = 0, nlk = 0, ntcol = 0, ntcolb = 0, ntgh = 0, ntlk = 0;
     int wide = 0, wide_g = 0;
 };
 
@@ -439,7 +440,9 @@ int step_level(lb_ctx* c, int li, int64_t j) {
             if (fill) HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 1, c->stream));
             hipEvent_t e2 = nullptr;
             RC(profile_begin(c, L, 1, &e2));
-            HIPCHK(c, c->k->thermal(las, st, L.wcol, L.nb, j, storage, L.wide_g ? 1 : 0, 0, c->stream));
+            HIPCHK(c, c->k->thermal(las, st, L.wcol, L.n_int, j, storage, L.wide_g ? 1 : 0, 0, 0, c->stream));
+            HIPCHK(c, c->k->thermal(las, st, L.wcol + L.n_int, L.nb - L.n_int, j, storage, L.wide_g ? 1 : 0, 1, 0,
+                                    c->stream));
             if (e2) HIPCHK(c, hipEventRecord(e2, c->stream));
             if (L.n_lchunks > 0) HIPCHK(c, c->k->bouzidi(las, st, L.wlk, L.n_lchunks, j, storage, 1, c->stream));
         }
@@ -624,7 +627,7 @@ int engine_rebuild(lb_engine* e) {
     RC(eupload(e, &e->states, sts));
     e->depth.assign((size_t)e->lmax, DepthWork());
     for (int d = 1; d <= e->lmax; d++) {
-        std::vector<lbk::Work> col, colb, gh, lk, tcol, tgh, tlk;
+        std::vector<lbk::Work> col, colb, gh, lk, tcol, tcolb, tgh, tlk;
         DepthWork& W = e->depth[(size_t)d - 1];
         for (size_t i = 0; i < e->cases.size(); i++) {
             const lb_ctx* c = e->cases[i].c;
@@ -638,9 +641,9 @@ int engine_rebuild(lb_engine* e) {
             for (size_t q = 0; q < lev.order_host.size(); q++) {
                 const int32_t b = lev.order_host[q];
                 const lbk::Work w = mkwork(li, b, ci, lev.nbr + (size_t)b * Q);
-                if ((int32_t)q < lev.n_int) col.push_back(w);
-                else colb.push_back(w);
-                if (ddf) tcol.push_back(w);
+                const bool interior = (int32_t)q < lev.n_int;
+                (interior ? col : colb).push_back(w);
+                if (ddf) (interior ? tcol : tcolb).push_back(w);
             }
             if (l > 1)
                 for (int64_t q = 0; q < lev.ngent; q += 256) {
@@ -659,6 +662,7 @@ int engine_rebuild(lb_engine* e) {
         RC(eupload(e, &W.gh, gh));
         RC(eupload(e, &W.lk, lk));
         RC(eupload(e, &W.tcol, tcol));
+        RC(eupload(e, &W.tcolb, tcolb));
         RC(eupload(e, &W.tgh, tgh));
         RC(eupload(e, &W.tlk, tlk));
         W.ncol = (int32_t)col.size();
@@ -666,6 +670,7 @@ int engine_rebuild(lb_engine* e) {
         W.ngh = (int32_t)gh.size();
         W.nlk = (int32_t)lk.size();
         W.ntcol = (int32_t)tcol.size();
+        W.ntcolb = (int32_t)tcolb.size();
         W.ntgh = (int32_t)tgh.size();
         W.ntlk = (int32_t)tlk.size();
     }
@@ -692,9 +697,11 @@ hipError_t engine_op(lb_engine* e, int d, int64_t j) {
     if (W.ncolb)
         ETRY(k->collide(e->las, st, W.colb, W.ncolb, j, e->storage, W.wide, 1, e->sched, e->waves, e->stream));
     if (W.nlk) ETRY(k->bouzidi(e->las, st, W.lk, W.nlk, j, e->storage, 0, e->stream));
-    if (W.ntcol) {
+    if (W.ntcol || W.ntcolb) {
         if (W.ntgh && fill) ETRY(k->ghost_fill(e->las, st, W.tgh, W.ntgh, j, e->storage, 1, e->stream));
-        ETRY(k->thermal(e->las, st, W.tcol, W.ntcol, j, e->storage, W.wide_g, e->sched, e->stream));
+        if (W.ntcol) ETRY(k->thermal(e->las, st, W.tcol, W.ntcol, j, e->storage, W.wide_g, 0, e->sched, e->stream));
+        if (W.ntcolb)
+            ETRY(k->thermal(e->las, st, W.tcolb, W.ntcolb, j, e->storage, W.wide_g, 1, e->sched, e->stream));
         if (W.ntlk) ETRY(k->bouzidi(e->las, st, W.tlk, W.ntlk, j, e->storage, 1, e->stream));
     }
     if (d < e->lmax) {

@@ -138,7 +138,8 @@ struct ForceArgs {
                        int32_t n_tiles, int64_t j, int storage, int wide, int boundary, int sched,       \
                        int waves, hipStream_t s);                                                        \
     hipError_t thermal(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work,    \
-                       int32_t n_tiles, int64_t j, int storage, int wide, int sched, hipStream_t s);    \
+                       int32_t n_tiles, int64_t j, int storage, int wide, int boundary, int sched,       \
+                       hipStream_t s);                                                                   \
     hipError_t ghost_fill(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work, int32_t n_chunks, int64_t j,  \
                           int storage, int thermal, hipStream_t s);                                      \
     hipError_t bouzidi(const lbk::LevelArgs* las, const lbk::CaseState* states, const lbk::Work* work, int32_t n_chunks, int64_t j,     \

@@ -943,14 +943,14 @@ __device__ __forceinline__ float thermal_bc(const ThermalArgs& a, int k, int b, 
     return w * a.T_inlet;
 }
 
-template <int ST, class OffT>
+template <int ST, class OffT, bool BND>
 __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restrict__ las,
                                                       const CaseState* __restrict__ states,
                                                       const Work* __restrict__ work, int32_t n_tiles, int64_t j,
                                                       int32_t sched) {
     __shared__ OffT ptab[108];
     __shared__ uint64_t smask[2];
-    __shared__ float sfix[27 * 256];
+    __shared__ float sfix[BND ? 27 * 256 : 1];
     Work wk;
     int half;
     if (!schedule(work, n_tiles, sched, wk, half)) return;
@@ -985,25 +985,28 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
                          (OffT)((512 * k - (cx + 8 * cy + 64 * cz)) * ESZ);
         G[k] = Codec<ST>::loadb(a.g_in, off, WK(k));
     }
-    uint32_t miss = 0u;
-    const uint32_t m = wave_miss(smask, wv);
-    if (m != 0u) {
+    if constexpr (BND) {
+        // boundary blocks only: sources in no block take the boundary cascade
+        uint32_t miss = 0u;
+        const uint32_t m = wave_miss(smask, wv);
+        if (m != 0u) {
 #pragma unroll
-        for (int k = 0; k < 27; k++) {
-            const int cx = CX(k), cy = CY(k), cz = CZ(k);
-            const int ox = cx == 1 ? oxp : (cx == -1 ? oxm : 0);
-            const int oy = cy == 1 ? oyp : (cy == -1 ? oym : 0);
-            miss |= ((m >> (9 * (cz + 1) + (ox + 1) + 3 * (oy + 1))) & 1u) << k;
+            for (int k = 0; k < 27; k++) {
+                const int cx = CX(k), cy = CY(k), cz = CZ(k);
+                const int ox = cx == 1 ? oxp : (cx == -1 ? oxm : 0);
+                const int oy = cy == 1 ? oyp : (cy == -1 ? oym : 0);
+                miss |= ((m >> (9 * (cz + 1) + (ox + 1) + 3 * (oy + 1))) & 1u) << k;
+            }
         }
-    }
-    if (miss != 0u) {
+        if (miss != 0u) {
 #pragma unroll 1
-        for (int k = 0; k < 27; k++)
-            if (miss & (1u << k))
-                sfix[k * 256 + threadIdx.x] = thermal_bc<ST>(a, k, b, c, gx - CX(k), gy - CY(k), gz - CZ(k));
+            for (int k = 0; k < 27; k++)
+                if (miss & (1u << k))
+                    sfix[k * 256 + threadIdx.x] = thermal_bc<ST>(a, k, b, c, gx - CX(k), gy - CY(k), gz - CZ(k));
 #pragma unroll
-        for (int k = 0; k < 27; k++)
-            if (miss & (1u << k)) G[k] = sfix[k * 256 + threadIdx.x];
+            for (int k = 0; k < 27; k++)
+                if (miss & (1u << k)) G[k] = sfix[k * 256 + threadIdx.x];
+        }
     }
     float Tsum = 0.0f;
 #pragma unroll
@@ -1046,19 +1049,28 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
     }
 }
 
+// Interior blocks (every neighbour present, ghosts included) run the variant
+// without the boundary cascade and its 27 KiB LDS buffer, which would limit
+// occupancy to 5 work-groups per CU.
+template <int ST, class OffT>
+void thermal_launch2(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_tiles, int64_t j,
+                     int boundary, int sched, hipStream_t s) {
+    const dim3 g(padded_groups(n_tiles)), t(256);
+    if (boundary) hipLaunchKernelGGL((thermal_kernel<ST, OffT, true>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+    else hipLaunchKernelGGL((thermal_kernel<ST, OffT, false>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+}
 template <int ST>
 void thermal_launch(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_tiles, int64_t j,
-                    int wide, int sched, hipStream_t s) {
-    const dim3 g(padded_groups(n_tiles)), t(256);
-    if (wide) hipLaunchKernelGGL((thermal_kernel<ST, uint64_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
-    else hipLaunchKernelGGL((thermal_kernel<ST, uint32_t>), g, t, 0, s, las, states, work, n_tiles, j, sched);
+                    int wide, int boundary, int sched, hipStream_t s) {
+    if (wide) thermal_launch2<ST, uint64_t>(las, states, work, n_tiles, j, boundary, sched, s);
+    else thermal_launch2<ST, uint32_t>(las, states, work, n_tiles, j, boundary, sched, s);
 }
 hipError_t thermal(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_tiles, int64_t j,
-                   int storage, int wide, int sched, hipStream_t s) {
+                   int storage, int wide, int boundary, int sched, hipStream_t s) {
     if (n_tiles <= 0) return hipSuccess;
-    if (storage == 1) thermal_launch<1>(las, states, work, n_tiles, j, wide, sched, s);
-    else if (storage == 2) thermal_launch<2>(las, states, work, n_tiles, j, wide, sched, s);
-    else thermal_launch<0>(las, states, work, n_tiles, j, wide, sched, s);
+    if (storage == 1) thermal_launch<1>(las, states, work, n_tiles, j, wide, boundary, sched, s);
+    else if (storage == 2) thermal_launch<2>(las, states, work, n_tiles, j, wide, boundary, sched, s);
+    else thermal_launch<0>(las, states, work, n_tiles, j, wide, boundary, sched, s);
     return hipGetLastError();
 }
 
