@@ -190,9 +190,19 @@ def get_active_blocks_for_level(mesh: SolverMesh, dx: float, offset, dims) -> np
     hi = np.floor((tmax + margin) * inv_bs_dx).astype(np.int64) + 1
     lo = np.maximum(lo, 1)
     hi = np.minimum(hi, np.asarray(dims))
-    for t in range(lo.shape[0]):
-        if (hi[t] >= lo[t]).all():
-            grid[lo[t, 0] - 1:hi[t, 0], lo[t, 1] - 1:hi[t, 1], lo[t, 2] - 1:hi[t, 2]] = True
+    keep = (hi >= lo).all(axis=1)
+    lo, hi = lo[keep] - 1, hi[keep] - 1          # 0-based, inclusive
+    span = hi - lo
+    # the union of the triangles' block boxes: boxes up to 4 blocks wide per
+    # axis are set in 64 vectorised passes, wider ones one by one
+    small = (span <= 3).all(axis=1)
+    for ox in range(4):
+        for oy in range(4):
+            for oz in range(4):
+                m = small & (span[:, 0] >= ox) & (span[:, 1] >= oy) & (span[:, 2] >= oz)
+                grid[lo[m, 0] + ox, lo[m, 1] + oy, lo[m, 2] + oz] = True
+    for t in np.nonzero(~small)[0]:
+        grid[lo[t, 0]:hi[t, 0] + 1, lo[t, 1]:hi[t, 1] + 1, lo[t, 2]:hi[t, 2] + 1] = True
     return grid
 
 
@@ -212,21 +222,26 @@ def _sibling_expand(g: np.ndarray) -> np.ndarray:
     pd = tuple((d + 1) // 2 for d in dims)
     pad = np.zeros(tuple(2 * p for p in pd), dtype=bool)
     pad[:dims[0], :dims[1], :dims[2]] = g
-    par = pad.reshape(pd[0], 2, pd[1], 2, pd[2], 2).any(axis=(1, 3, 5))
-    full = np.repeat(np.repeat(np.repeat(par, 2, 0), 2, 1), 2, 2)
-    return full[:dims[0], :dims[1], :dims[2]]
+    par = np.zeros(pd, dtype=bool)
+    for i in (0, 1):
+        for j in (0, 1):
+            for k in (0, 1):
+                par |= pad[i::2, j::2, k::2]
+    full = np.broadcast_to(par[:, None, :, None, :, None], (pd[0], 2, pd[1], 2, pd[2], 2))
+    return full.reshape(2 * pd[0], 2 * pd[1], 2 * pd[2])[:dims[0], :dims[1], :dims[2]]
 
 
 def add_halo_blocks_with_siblings(active: np.ndarray, layers: int) -> np.ndarray:
     """add_halo_blocks_with_siblings! (domain_topology.jl:54-99)."""
     for _ in range(layers):
-        dil = np.zeros_like(active)
-        for dz in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dx in (-1, 0, 1):
-                    if dx == 0 and dy == 0 and dz == 0:
-                        continue
-                    dil |= _shift_or(active, dx, dy, dz)
+        # 26-neighbourhood dilation, as three separable 1-D dilations
+        dil = active
+        for ax in range(3):
+            d = [0, 0, 0]
+            d[ax] = 1
+            fwd = _shift_or(dil, *d)
+            d[ax] = -1
+            dil = dil | fwd | _shift_or(dil, *d)
         new = dil & ~active
         sib = _sibling_expand(new) & ~active & ~new
         active = active | new | sib
@@ -347,13 +362,12 @@ def setup_multilevel_domain(cfg: CaseConfig, mesh: SolverMesh, params: DomainPar
                 ov = (bmin[:, 0] <= wake_end_x) & (bmax[:, 0] >= wake_start_x) & \
                      (bmin[:, 1] <= wmax_y) & (bmax[:, 1] >= wmin_y) & \
                      (bmin[:, 2] <= wmax_z) & (bmax[:, 2] >= wmin_z)
-                for cb in prev_coords[ov]:
-                    for dbz in (0, 1):
-                        for dby in (0, 1):
-                            for dbx in (0, 1):
-                                f = (2 * cb[0] - 1 + dbx, 2 * cb[1] - 1 + dby, 2 * cb[2] - 1 + dbz)
-                                if 1 <= f[0] <= dims[0] and 1 <= f[1] <= dims[1] and 1 <= f[2] <= dims[2]:
-                                    active[f[0] - 1, f[1] - 1, f[2] - 1] = True
+                # the 8 children of every previous-level block the wake box touches
+                kids = np.array([(dbx, dby, dbz) for dbz in (0, 1) for dby in (0, 1) for dbx in (0, 1)])
+                f = (2 * prev_coords[ov].astype(np.int64) - 1)[:, None, :] + kids[None, :, :]
+                f = f.reshape(-1, 3)
+                f = f[((f >= 1) & (f <= np.asarray(dims))).all(axis=1)]
+                active[f[:, 0] - 1, f[:, 1] - 1, f[:, 2] - 1] = True
             # orphan filter: parent (b+1)÷2 must be active on the previous level
             pexp = np.repeat(np.repeat(np.repeat(prev_grid, 2, 0), 2, 1), 2, 2)[:dims[0], :dims[1], :dims[2]]
             active &= pexp
