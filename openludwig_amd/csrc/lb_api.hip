@@ -852,7 +852,11 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
         borg.push_back((int32_t)((py - 1) * 8));
         borg.push_back((int32_t)((pz - 1) * 8));
         // ghost cell s serves direction k when s + c_k lies in a block of the level
-        for (int cl = 0; cl < NC; cl++) {
+        uint32_t has_real = 0;  // bit (ox+1) + 3(oy+1) + 9(oz+1): that neighbour block is real
+        for (int n = 0; n < Q; n++)
+            if (n != 13 && real.count(key(px + n % 3 - 1, py + (n / 3) % 3 - 1, pz + n / 9 - 1)))
+                has_real |= 1u << n;
+        for (int cl = 0; cl < NC && has_real != 0u; cl++) {
             const int lx = cl & 7, ly = (cl >> 3) & 7, lz = cl >> 6;
             uint32_t mask = 0;
             for (int k = 0; k < Q; k++) {
@@ -860,7 +864,7 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
                 const int ox = nx < 0 ? -1 : (nx > 7 ? 1 : 0), oy = ny < 0 ? -1 : (ny > 7 ? 1 : 0),
                           oz = nz < 0 ? -1 : (nz > 7 ? 1 : 0);
                 if (ox == 0 && oy == 0 && oz == 0) continue;
-                if (real.count(key(px + ox, py + oy, pz + oz))) mask |= 1u << k;
+                if (has_real & (1u << ((ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)))) mask |= 1u << k;
             }
             if (mask) {
                 gent.push_back((int32_t)(((nb + g) << 9) | cl));
