@@ -908,20 +908,22 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     if (nb > 0) {
         RC(h2d(c, L.nbr, nbr.data(), nbr.size()));
         RC(h2d(c, L.borg, borg.data(), borg.size()));
-        // processing order: z-layers of blocks, and within a layer the 2-D
-        // Morton order of (x, y).  The x and y neighbours of a block, whose
-        // faces cut 128-byte lines (a line is 4 rows of 8 cells of one
-        // z-plane), are then pulled while they are in L2; z faces are whole
-        // lines.  Measured on the 52.5M-cell cube case against the 3-D Morton
-        // order: HBM reads of the finest collide 146.8 instead of 162.6 B/cell
-        // (FP32), 87.0 instead of 95.5 (FP16).
+        // processing order: slabs of 4 block layers in z; inside a slab the
+        // 2-D Morton order of the (x, y) columns, the 4 blocks of a column
+        // consecutive.  x and y faces cut 128-byte lines (a line is 4 rows of
+        // 8 cells of one z-plane), so x/y neighbours should be pulled while
+        // their lines are in L2; the columns also keep z neighbours close.
+        // Measured on the 52.5M-cell cube case, finest collide: 3-D Morton
+        // 1866 us, z-layers with 2-D Morton 1850 us, these slabs 1818 us (FP32;
+        // FP16 1216 / 1207 / 1193 us).
         std::vector<std::pair<uint64_t, int32_t>> key((size_t)nb);
         for (int32_t b = 0; b < nb; b++) {
             uint64_t m = 0;
             const uint32_t vx = (uint32_t)(d->map_x[b] - 1), vy = (uint32_t)(d->map_y[b] - 1);
+            const uint32_t vz = (uint32_t)(d->map_z[b] - 1);
             for (int bit = 0; bit < 21; bit++)
                 m |= (uint64_t)((vx >> bit) & 1u) << (2 * bit) | (uint64_t)((vy >> bit) & 1u) << (2 * bit + 1);
-            key[(size_t)b] = {((uint64_t)(d->map_z[b] - 1) << 42) | m, b};
+            key[(size_t)b] = {((uint64_t)(vz >> 2) << 50) | (m << 8) | (vz & 3u), b};
         }
         std::sort(key.begin(), key.end());
         // interior blocks (all 26 neighbours present, ghosts included) first,

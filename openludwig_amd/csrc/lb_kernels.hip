@@ -395,8 +395,8 @@ __device__ __forceinline__ void chim_bwd(float& k0, float& k1, float& k2, float 
 
 // Work-group -> block schedule.  Work-groups are dealt round-robin to the 8
 // XCDs (each with its own L2); the swizzle gives every XCD one contiguous run
-// of the level's processing order (z-layers, 2-D Morton order inside, two
-// work-groups per block), so a block and most of its neighbours share an L2
+// of the level's processing order (4-layer z-slabs, 2-D Morton order of the
+// (x, y) columns inside, two work-groups per block), so a block and most of its neighbours share an L2
 // and are processed close in time.  Returns false for padding work-groups.
 // sched = 1 maps work-group i to half-tile i (no XCD awareness).
 __device__ __forceinline__ bool schedule(const Work* __restrict__ work, int n_tiles, int sched, Work& wk,
