@@ -1,4 +1,4 @@
-"""Writes c1_sphere_re1e3_fp64.json: the FP64 oracle's force history of the
+"""Writes sphere_re1e3_fp64_2000steps.json: the FP64 oracle's force history of the
 plain CPU-runnable sphere case (D = 1 m icosphere subdiv 4, Re 1e3, coarse
 64x32x32, 3 levels, 2000 coarse steps, ramp 500, batches of 8, diagnostics
 every 100 steps), stepped with the main loop's cadence (main.jl:208-260: the
@@ -6,7 +6,7 @@ inlet velocity of each batch evaluated at its last step, forces at the batch
 that contains a diagnostic step, the average over diagnostic steps after the
 ramp).  The GPU test compares the library's run of the same case with it.
 
-    python tests/golden/make_c1_fp64.py [--steps N]      (about 30 min on 8 cores)
+    python tests/golden/make_sphere_re1e3_fp64.py [--steps N]      (about 30 min on 8 cores)
 """
 from __future__ import annotations
 
@@ -28,7 +28,7 @@ from oracle import Oracle, oracle_forces  # noqa: E402
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--out", default=str(Path(__file__).with_name("c1_sphere_re1e3_fp64.json")))
+    ap.add_argument("--out", default=str(Path(__file__).with_name("sphere_re1e3_fp64_2000steps.json")))
     a = ap.parse_args()
     cfg, mesh = cases.sphere_re1e3()
     grids, params, mesh = build_case(cfg, mesh)

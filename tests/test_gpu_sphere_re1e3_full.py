@@ -2,7 +2,7 @@
 (icosphere subdiv 4), Re 1e3, coarse 64x32x32, 3 levels, 2000 coarse steps
 with the main loop's cadence (ramp 500, batches of 8, forces every 100
 steps), against the FP64 oracle's run of the same case, recorded in
-golden/c1_sphere_re1e3_fp64.json by golden/make_c1_fp64.py.
+golden/sphere_re1e3_fp64_2000steps.json by golden/make_sphere_re1e3_fp64.py.
 
 The wake loses its symmetry after ~700 steps and sheds (Cl swings between
 +0.28 and -0.22 by step 2000); the FP32 run follows the FP64 oracle through
@@ -24,13 +24,13 @@ from openludwig_amd import _native as N
 from openludwig_amd import cases
 from openludwig_amd.solver import Solver, build_case
 
-FIXTURE = Path(__file__).parent / "golden" / "c1_sphere_re1e3_fp64.json"
+FIXTURE = Path(__file__).parent / "golden" / "sphere_re1e3_fp64_2000steps.json"
 
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not FIXTURE.is_file(), reason="FP64 reference run not recorded")
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
-def test_c1_full_run_vs_fp64_oracle(hip_device, storage):
+def test_sphere_re1e3_full_run_vs_fp64_oracle(hip_device, storage):
     ref = json.loads(FIXTURE.read_text())
     cfg, mesh = cases.sphere_re1e3()
     grids, params, mesh = build_case(cfg, mesh)
@@ -47,7 +47,7 @@ def test_c1_full_run_vs_fp64_oracle(hip_device, storage):
     for h in ref["history"]:
         g = hist[h["step"]]
         rows.append((h["step"], g["Cd"], h["Cd"], g["Cl"], h["Cl"]))
-    print(f"\nC1 {storage}: step  Cd gpu / fp64  Cl gpu / fp64")
+    print(f"\nsphere Re 1e3, {storage}: step  Cd gpu / fp64  Cl gpu / fp64")
     for r in rows:
         print("  %5d  %.6f / %.6f  %+.6f / %+.6f" % r)
     print(f"  post-ramp average: Cd {run['Cd_avg']:.6f} vs {ref['Cd_avg']:.6f}, "

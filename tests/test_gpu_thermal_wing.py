@@ -1,4 +1,4 @@
-"""Compressible thermal form on a wing (config 5 in small): the extruded
+"""Compressible thermal form on a wing, at a small size: the extruded
 NACA 0012 at alpha 2 deg with the thermal DDF and the O(u^4) equilibrium,
 2 levels at a resolution the CPU oracle runs in seconds.  The fast GPU build
 against the FP64 oracle after 96 coarse steps (FP32 and FP16 storage): Cl and Cd within
