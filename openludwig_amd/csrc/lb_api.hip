@@ -76,6 +76,7 @@ struct Level {
     uint8_t* obst = nullptr;
     float* sponge = nullptr;
     float* wall_dist = nullptr;
+    uint8_t* bflags = nullptr;  // per block: any sponge / near-wall / solid cell
     int32_t* nbr = nullptr;
     int32_t* borg = nullptr;
     int32_t* order = nullptr;
@@ -518,6 +519,7 @@ int upload_level_args(lb_ctx* c, int li) {
     A.obst = L.obst;
     A.sponge = L.sponge;
     A.wall_dist = L.wall_dist;
+    A.bflags = L.bflags;
     A.nbr = L.nbr;
     A.borg = L.borg;
     A.nb = L.nb;
@@ -895,6 +897,7 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     RC(dalloc(c, &L.obst, n4));
     RC(dalloc(c, &L.sponge, n4));
     if (d->wall_dist) RC(dalloc(c, &L.wall_dist, n4));
+    RC(dalloc(c, &L.bflags, (size_t)nb));
     RC(dalloc(c, &L.nbr, (size_t)nb * Q));
     RC(dalloc(c, &L.borg, borg.size()));
     RC(dalloc(c, &L.order, (size_t)nb));
@@ -954,6 +957,18 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
         RC(h2d(c, L.obst, d->obstacle, n4));
         RC(h2d(c, L.sponge, d->sponge, n4));
         if (d->wall_dist) RC(h2d(c, L.wall_dist, d->wall_dist, n4));
+        std::vector<uint8_t> bfl((size_t)nb, 0);
+        for (int32_t b = 0; b < nb; b++) {
+            uint8_t f = 0;
+            for (int i = 0; i < NC; i++) {
+                const size_t k = (size_t)b * NC + i;
+                if (d->sponge[k] > 0.0f) f |= 1;
+                if (d->wall_dist && d->wall_dist[k] > 0.0f && d->wall_dist[k] < 10.0f) f |= 2;
+                if (d->obstacle[k]) f |= 4;
+            }
+            bfl[(size_t)b] = f;
+        }
+        RC(h2d(c, L.bflags, bfl.data(), bfl.size()));
     }
     if (ng > 0) {
         std::vector<int32_t> grid(ng);

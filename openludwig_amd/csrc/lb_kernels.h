@@ -67,6 +67,9 @@ struct LevelArgs {
     const uint8_t* obst;
     const float* sponge;
     const float* wall_dist;  // null when the level has no wall distances
+    // per block: bit 0 some cell has sponge > 0; bit 1 some cell has
+    // 0 < wall distance < 10 (the wall model's range); bit 2 some cell is solid
+    const uint8_t* bflags;
     const int32_t* nbr;
     const int32_t* borg;
     int32_t nb;
