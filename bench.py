@@ -214,7 +214,11 @@ def main() -> int:
                    "mlups_ref": round(m_ref, 1), "build": "strict" if args.strict else "fast",
                    "population_storage": args.storage, "bytes_per_lu": b_alg,
                    "hbm_roofline_pct_step": round(m_true * 1e6 * b_alg / (HBM_PEAK_GBS * 1e9) * 100, 2),
-                   "per_level_collide": per_level, "Cd": round(forces["Cd"], 5), "thermal": thermal},
+                   "per_level_collide": per_level,
+                   # a sanity figure early in the inflow ramp (coarse step warmup + steps of
+                   # ramp_steps), normalised by the full inflow velocity: small by design
+                   "Cd": float(f"{forces['Cd']:.6g}"), "Cd_at_coarse_step": args.warmup + args.steps,
+                   "ramp_steps": cfg.ramp_steps, "thermal": thermal},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "collide_kernel (finest level)", "bytes_per_launch": bytes_launch,
