@@ -17,7 +17,9 @@ steps, with the populations resident in HBM.  roofline = the finest level's
 stream-collide kernel (the dominant one): algorithmic bytes per launch
 (253 B per cell update: 2x27x4 B populations + 33 B macroscopic/flags + 4 B
 wall distance; 145 B with --storage fp16/bf16, populations as 16-bit offsets
-f - w) / its average launch time from HIP events on the library's stream.
+f - w; on the finest level the 4 B density store happens on one substep of
+the 8 per coarse step, so its launches average 249.5 and 141.5 B) / its
+average launch time from HIP events on the library's stream.
 cpu_baseline = the C oracle (the restated reference algorithm) on the host
 cores over one coarse step of the same workload.  serving = the serving
 comparison of openludwig_amd.serving.compare (FP16 continuous batching vs
@@ -38,6 +40,15 @@ sys.path.insert(0, str(ROOT))
 # bytes per cell update with the wall model: 2 x 27 populations + 33 B of
 # macroscopic fields/flags + 4 B wall distance
 B_ALG_WMLES = {"fp32": 2 * 27 * 4 + 37, "fp16": 2 * 27 * 2 + 37, "bf16": 2 * 27 * 2 + 37}
+
+
+def leaf_bytes(b: float, n_levels: int) -> float:
+    """Average bytes per cell update of a finest-level launch: the 4-byte
+    density (or temperature) store happens on the last of the level's
+    2^(n_levels - 1) substeps per coarse step only."""
+    return b - 4.0 * (1.0 - 1.0 / 2 ** (n_levels - 1))
+
+
 STORAGE_CODE = {"fp32": 0, "fp16": 1, "bf16": 2}
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 
@@ -140,16 +151,19 @@ def main() -> int:
         # thermal DDF kernel: 2x27 populations (4 or 2 B) + u 12 + T 4 + obstacle 1 + sponge 4
         tms, tn = kern_t[finest]
         t_avg = tms / max(tn, 1)
-        tb = 2 * 27 * (4 if args.storage == "fp32" else 2) + 21
+        tb = leaf_bytes(2 * 27 * (4 if args.storage == "fp32" else 2) + 21, len(grids))
         thermal = {"bytes_per_lu": tb, "finest_avg_launch_ms": round(t_avg, 4),
                    "finest_GBps": round(grids[-1].n_cells * tb / (t_avg * 1e-3) / 1e9, 1)}
     ms_k, n_k = kern[finest]
     avg_ms = ms_k / max(n_k, 1)
-    bytes_launch = grids[-1].n_cells * b_alg
+    b_lev = {g.level_id: (leaf_bytes(b_alg, len(grids)) if g.level_id == finest else b_alg) for g in grids}
+    bytes_launch = round(grids[-1].n_cells * b_lev[finest])
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    per_level = {str(lid): {"launches": n, "avg_ms": round(ms / max(n, 1), 4),
-                            "GBps": round(grids[lid - 1].n_cells * b_alg / (ms / max(n, 1) * 1e-3) / 1e9, 1)
+    per_level = {str(lid): {"launches": n, "avg_ms": round(ms / max(n, 1), 4), "bytes_per_lu": b_lev[lid],
+                            "GBps": round(grids[lid - 1].n_cells * b_lev[lid] / (ms / max(n, 1) * 1e-3) / 1e9, 1)
                             if n else None} for lid, (ms, n) in kern.items()}
+    step_bytes = sum(g.n_cells * 2 ** (g.level_id - 1) * b_lev[g.level_id] for g in grids)
+    step_lu = sum(g.n_cells * 2 ** (g.level_id - 1) for g in grids)
     log(f"timed {args.steps} coarse steps in {elapsed:.3f}s: MLUPS_true {m_true:.0f}, MLUPS_ref {m_ref:.0f}; "
         f"finest collide {avg_ms:.3f} ms/launch = {achieved:.0f} GB/s; Cd {forces['Cd']:.4f}")
 
@@ -212,8 +226,8 @@ def main() -> int:
                                f"resolution {args.resolution}",
                    "cells_per_level": n_cells, "lu_per_step": sum(c * 2 ** i for i, c in enumerate(n_cells)),
                    "mlups_ref": round(m_ref, 1), "build": "strict" if args.strict else "fast",
-                   "population_storage": args.storage, "bytes_per_lu": b_alg,
-                   "hbm_roofline_pct_step": round(m_true * 1e6 * b_alg / (HBM_PEAK_GBS * 1e9) * 100, 2),
+                   "population_storage": args.storage, "bytes_per_lu": round(step_bytes / step_lu, 2),
+                   "hbm_roofline_pct_step": round(m_true * 1e6 * step_bytes / step_lu / (HBM_PEAK_GBS * 1e9) * 100, 2),
                    "per_level_collide": per_level,
                    # a sanity figure early in the inflow ramp (coarse step warmup + steps of
                    # ramp_steps), normalised by the full inflow velocity: small by design

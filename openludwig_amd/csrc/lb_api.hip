@@ -528,6 +528,7 @@ int upload_level_args(lb_ctx* c, int li) {
     A.nyg = c->cfg.domain_ny * scale;
     A.nzg = c->cfg.domain_nz * scale;
     A.lshift = L.id - 1;
+    A.leaf = li + 1 == (int)c->levels.size() ? 1 : 0;
     A.tau = L.tau;
     A.tau_g = L.tau_g;
     if (li > 0) {

@@ -75,6 +75,10 @@ struct LevelArgs {
     int32_t nb;
     int32_t nxg, nyg, nzg;
     int32_t lshift;  // level_id - 1
+    // 1 for the finest level (no child level reads its density or
+    // temperature): those scalar fields are then stored only on the last
+    // substep of each coarse step, the only one whose output is read
+    int32_t leaf;
     float tau, tau_g;
     // parent level (level_id > 1): its pairs, indexed by the parent's parity
     const void* pf[2];

@@ -71,6 +71,7 @@ struct CollideArgs {
     int32_t adaptive, limiter, comp, wall_model, sponge_blend;
     int32_t seed;
     int32_t is_l1;
+    int32_t store_rho;  // store rho_out on this substep
 };
 
 struct ThermalArgs {
@@ -88,6 +89,7 @@ struct ThermalArgs {
     float u_inlet, T_inlet, T_wall;
     int32_t adiabatic, sponge_blend;
     int32_t is_l1;
+    int32_t store_T;  // store T_out on this substep
 };
 
 struct GhostArgs {
@@ -141,6 +143,17 @@ __device__ __forceinline__ float inlet_u(const LevelArgs& L, const CaseState& S,
     return ((const __attribute__((address_space(1))) float*)S.u_tab)[b];
 }
 
+// Whether substep j stores the level's density (flow) or temperature
+// (thermal) output.  A coarser level's are read by the ghost fill of its
+// child on every substep; the finest level's only by forces, statistics and
+// downloads, which run between coarse steps, so there only the last substep
+// of each coarse step stores them (4 of the 125 bytes written per cell in
+// FP32).
+__device__ __forceinline__ int32_t store_scalars(const LevelArgs& L, int64_t j) {
+    const int64_t m = ((int64_t)1 << L.lshift) - 1;
+    return (L.leaf == 0 || (j & m) == m) ? 1 : 0;
+}
+
 __device__ __forceinline__ CollideArgs collide_view(const LevelArgs& L, const CaseState& S, int64_t j) {
     const int64_t ts = substep(L, S, j);
     const int par = (int)(ts & 1);
@@ -176,6 +189,7 @@ __device__ __forceinline__ CollideArgs collide_view(const LevelArgs& L, const Ca
     a.sponge_blend = S.sponge_blend;
     a.seed = (int32_t)(ts % 1000000);
     a.is_l1 = L.lshift == 0 ? 1 : 0;
+    a.store_rho = store_scalars(L, j);
     return a;
 }
 
@@ -201,6 +215,7 @@ __device__ __forceinline__ ThermalArgs thermal_view(const LevelArgs& L, const Ca
     t.adiabatic = S.adiabatic;
     t.sponge_blend = S.sponge_blend;
     t.is_l1 = L.lshift == 0 ? 1 : 0;
+    t.store_T = store_scalars(L, j);
     return t;
 }
 
@@ -640,7 +655,7 @@ collide_kernel(const LevelArgs* __restrict__ las,
         vo[iv] = 0.0f;
         vo[iv + NC] = 0.0f;
         vo[iv + 2 * NC] = 0.0f;
-        ro[i4] = 1.0f;
+        if (a.store_rho) ro[i4] = 1.0f;
 #pragma unroll
         for (int k = 0; k < 27; k++) Codec<ST>::stream(fo, fidx(b, k, c), A[OPP(k)], WK(k));
         return;
@@ -701,7 +716,7 @@ collide_kernel(const LevelArgs* __restrict__ las,
         __builtin_nontemporal_store(ux, &vo[iv]);
         __builtin_nontemporal_store(uy, &vo[iv + NC]);
         __builtin_nontemporal_store(uz, &vo[iv + 2 * NC]);
-        __builtin_nontemporal_store(rho, &ro[i4]);
+        if (a.store_rho) __builtin_nontemporal_store(rho, &ro[i4]);
     }
 
     // STEP 3: WALE on the previous velocity (kernel_cumulant.jl:264-310)
@@ -1039,7 +1054,7 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
             }
             Codec<ST>::stream(a.g_out, fidx(b, k, c), go, WK(k));
         }
-        a.T_out[i4] = a.adiabatic == 1 ? Tsum : a.T_wall;
+        if (a.store_T) a.T_out[i4] = a.adiabatic == 1 ? Tsum : a.T_wall;
         return;
     }
     float Tl = fmaxf(Tsum, 0.01f);
@@ -1050,7 +1065,7 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
         uy = uy * (1.0f - sp);
         uz = uz * (1.0f - sp);
     }
-    a.T_out[i4] = Tl;
+    if (a.store_T) a.T_out[i4] = Tl;
     const float omg = 1.0f / fmaxf(a.tau_g, 0.500001f);
 #pragma unroll
     for (int k = 0; k < 27; k++) {
