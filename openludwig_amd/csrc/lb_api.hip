@@ -919,7 +919,9 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
         // their lines are in L2; the columns also keep z neighbours close.
         // Measured on the 52.5M-cell cube case, finest collide: 3-D Morton
         // 1866 us, z-layers with 2-D Morton 1850 us, these slabs 1818 us (FP32;
-        // FP16 1216 / 1207 / 1193 us).
+        // FP16 1216 / 1207 / 1193 us).  A 2-D Hilbert curve in place of the
+        // Morton order, and x-strips 8-32 columns wide swept along y (slabs of
+        // 2, 4 or 8 layers), were all within 0.5 % of these slabs.
         std::vector<std::pair<uint64_t, int32_t>> key((size_t)nb);
         for (int32_t b = 0; b < nb; b++) {
             uint64_t m = 0;
