@@ -19,6 +19,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 #include "lb_kernels.h"
 
@@ -260,6 +261,8 @@ __device__ __forceinline__ const GF* gp(const float* p) { return (const GF*)p; }
 __device__ __forceinline__ GF* gp(float* p) { return (GF*)p; }
 __device__ __forceinline__ const GU8* gp(const uint8_t* p) { return (const GU8*)p; }
 typedef __attribute__((address_space(1))) uint16_t GU16;
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) v4u GV4U;
 typedef __attribute__((address_space(1))) char GC;
 
 // Population storage.  ST = 0: FP32.  ST = 1 / 2: the offset f - w_k in
@@ -282,6 +285,8 @@ struct Codec<0> {
     static __device__ __forceinline__ void stream(void* base, size_t i, float f, float) {
         __builtin_nontemporal_store(f, &((GF*)base)[i]);
     }
+    // the stored bits of f (for the staged stores of stream_tile)
+    static __device__ __forceinline__ uint32_t bits(float f, float) { return __float_as_uint(f); }
 };
 template <>
 struct Codec<1> {
@@ -299,6 +304,9 @@ struct Codec<1> {
     }
     static __device__ __forceinline__ void stream(void* base, size_t i, float f, float w) {
         __builtin_nontemporal_store((uint16_t)__half_as_ushort(__float2half_rn(f - w)), &((GU16*)base)[i]);
+    }
+    static __device__ __forceinline__ uint16_t bits(float f, float w) {
+        return (uint16_t)__half_as_ushort(__float2half_rn(f - w));
     }
 };
 template <>
@@ -324,7 +332,39 @@ struct Codec<2> {
     static __device__ __forceinline__ void stream(void* base, size_t i, float f, float w) {
         __builtin_nontemporal_store(enc(f - w), &((GU16*)base)[i]);
     }
+    static __device__ __forceinline__ uint16_t bits(float f, float w) { return enc(f - w); }
 };
+
+// Staged stores of a wave's stream-collide outputs.  A wave is one z-plane
+// (64 consecutive cells) of its block, so row k of its output is 64
+// consecutive slots of f[b][k][*].  Each lane writes its cell's 27 stored
+// values into the wave's [27][64] tile in LDS; then each lane stores 16
+// bytes of one row, so the outputs leave in 7 (FP32) or 4 (16-bit) vector
+// stores per wave instead of 27 scalar ones.  The LDS operations of one wave
+// complete in order, so the two phases need no barrier.
+template <int ST>
+using Stored = std::conditional_t<ST == 0, uint32_t, uint16_t>;
+
+template <int ST>
+__device__ __forceinline__ void stream_tile(void* fo, int b, int c, const float (&A)[27], Stored<ST>* tile) {
+    constexpr int EPL = 16 / (int)sizeof(Stored<ST>);  // values per 16-byte store
+    constexpr int LPR = 64 / EPL;                      // lanes per row of 64 values
+    const int lane = c & 63;
+#pragma unroll
+    for (int k = 0; k < 27; k++) tile[k * 64 + lane] = Codec<ST>::bits(A[k], WK(k));
+    __builtin_amdgcn_wave_barrier();
+    const int sub = lane % LPR;
+    const int c0 = c & ~63;
+#pragma unroll
+    for (int i = 0; i < (27 + EPL - 1) / EPL; i++) {
+        const int k = EPL * i + lane / LPR;
+        if (k < 27) {
+            const v4u v = *(const v4u*)(tile + k * 64 + sub * EPL);
+            const size_t slot = fidx(b, k, c0) + (size_t)(sub * EPL);
+            __builtin_nontemporal_store(v, (GV4U*)((GC*)fo + slot * sizeof(Stored<ST>)));
+        }
+    }
+}
 
 // Equilibria (src/collision/physics_utils.jl:19-82)
 __device__ __forceinline__ float feq2(float rho, float ux, float uy, float uz, float w, float cx, float cy, float cz) {
@@ -535,6 +575,10 @@ collide_kernel(const LevelArgs* __restrict__ las,
     __shared__ OffT vtab[32];
     __shared__ uint64_t smask[2];
     __shared__ float sfix[BND ? 27 * 256 : 1];
+    // interior blocks: outputs leave through stream_tile (the boundary
+    // kernel's cascade buffer already takes the LDS budget)
+    constexpr bool STAGE = !BND;
+    __shared__ __attribute__((aligned(16))) Stored<ST> sst[STAGE ? 4 * 27 * 64 : 8];
     Work wk;
     int half;
     if (!schedule(work, n_tiles, sched, wk, half)) return;
@@ -656,276 +700,292 @@ collide_kernel(const LevelArgs* __restrict__ las,
         vo[iv + NC] = 0.0f;
         vo[iv + 2 * NC] = 0.0f;
         if (a.store_rho) ro[i4] = 1.0f;
+        if constexpr (STAGE) {
 #pragma unroll
-        for (int k = 0; k < 27; k++) Codec<ST>::stream(fo, fidx(b, k, c), A[OPP(k)], WK(k));
-        return;
-    }
-
-    // STEP 2: macroscopic + sponge + wall model (kernel_cumulant.jl:190-259)
-    rho = fmaxf(rho, 0.01f);
-    const float inv_rho = 1.0f / rho;
-    float ux = jx * inv_rho, uy = jy * inv_rho, uz = jz * inv_rho;
-    if (sp > 0.0f) {
-        rho = rho * (1.0f - sp) + 1.0f * sp;
-        ux = ux * (1.0f - sp) + a.u_inlet * sp;
-        uy = uy * (1.0f - sp);
-        uz = uz * (1.0f - sp);
-        if (a.sponge_blend == 1) {
-#pragma unroll
-            for (int k = 0; k < 27; k++) {
-                float ft = feq_auto(1.0f, a.u_inlet, 0.0f, 0.0f, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k),
-                                    a.comp);
-                A[k] = A[k] * (1.0f - sp) + ft * sp;
+            for (int k = 0; k < 13; k++) {
+                const float t = A[k];
+                A[k] = A[OPP(k)];
+                A[OPP(k)] = t;
             }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 27; k++) Codec<ST>::stream(fo, fidx(b, k, c), A[OPP(k)], WK(k));
+            return;
         }
     }
-    float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
-    if (a.wall_model == 1) {
-        const float d = wdist;
-        if (d > 0.0f && d < 10.0f) {
-            float umag = sqrtf(ux * ux + uy * uy + uz * uz);
-            float nu_visc = (a.tau - 0.5f) / 3.0f;
-            if (umag > 1.0e-6f && nu_visc > 1.0e-10f) {
-                float u_tau = umag * powf(nu_visc / (d * umag + 1.0e-10f), 1.0f / 7.0f) *
-                              powf(2.0f * 8.3f, -1.0f / 7.0f);
-                u_tau = fmaxf(u_tau, 1.0e-6f);
-                float y_p = u_tau * d / nu_visc;
-                if (y_p > 11.81f) {
-                    float upl = (1.0f / 0.41f) * logf(y_p) + 5.2f;
-                    if (upl > 0.1f) {
-                        u_tau = u_tau * ((umag / u_tau) / upl);
-                        u_tau = fmaxf(u_tau, 1.0e-6f);
+    // (staged form: solid cells skip the collision and join the store)
+    if (!is_obs) {
+
+        // STEP 2: macroscopic + sponge + wall model (kernel_cumulant.jl:190-259)
+        rho = fmaxf(rho, 0.01f);
+        const float inv_rho = 1.0f / rho;
+        float ux = jx * inv_rho, uy = jy * inv_rho, uz = jz * inv_rho;
+        if (sp > 0.0f) {
+            rho = rho * (1.0f - sp) + 1.0f * sp;
+            ux = ux * (1.0f - sp) + a.u_inlet * sp;
+            uy = uy * (1.0f - sp);
+            uz = uz * (1.0f - sp);
+            if (a.sponge_blend == 1) {
+#pragma unroll
+                for (int k = 0; k < 27; k++) {
+                    float ft = feq_auto(1.0f, a.u_inlet, 0.0f, 0.0f, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k),
+                                        a.comp);
+                    A[k] = A[k] * (1.0f - sp) + ft * sp;
+                }
+            }
+        }
+        float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
+        if (a.wall_model == 1) {
+            const float d = wdist;
+            if (d > 0.0f && d < 10.0f) {
+                float umag = sqrtf(ux * ux + uy * uy + uz * uz);
+                float nu_visc = (a.tau - 0.5f) / 3.0f;
+                if (umag > 1.0e-6f && nu_visc > 1.0e-10f) {
+                    float u_tau = umag * powf(nu_visc / (d * umag + 1.0e-10f), 1.0f / 7.0f) *
+                                  powf(2.0f * 8.3f, -1.0f / 7.0f);
+                    u_tau = fmaxf(u_tau, 1.0e-6f);
+                    float y_p = u_tau * d / nu_visc;
+                    if (y_p > 11.81f) {
+                        float upl = (1.0f / 0.41f) * logf(y_p) + 5.2f;
+                        if (upl > 0.1f) {
+                            u_tau = u_tau * ((umag / u_tau) / upl);
+                            u_tau = fmaxf(u_tau, 1.0e-6f);
+                        }
+                    }
+                    float tau_wall = rho * u_tau * u_tau;
+                    float tau_res = rho * nu_visc * (umag / d);
+                    if (tau_wall > tau_res) {
+                        float fm = (tau_wall - tau_res) / d;
+                        Fx = -fm * ux / umag;
+                        Fy = -fm * uy / umag;
+                        Fz = -fm * uz / umag;
                     }
                 }
-                float tau_wall = rho * u_tau * u_tau;
-                float tau_res = rho * nu_visc * (umag / d);
-                if (tau_wall > tau_res) {
-                    float fm = (tau_wall - tau_res) / d;
-                    Fx = -fm * ux / umag;
-                    Fy = -fm * uy / umag;
-                    Fz = -fm * uz / umag;
+            }
+        }
+        const float uxe = ux + 0.5f * Fx * inv_rho;
+        const float uye = uy + 0.5f * Fy * inv_rho;
+        const float uze = uz + 0.5f * Fz * inv_rho;
+        {
+            const size_t iv = (size_t)b * 3 * NC + c;
+            __builtin_nontemporal_store(ux, &vo[iv]);
+            __builtin_nontemporal_store(uy, &vo[iv + NC]);
+            __builtin_nontemporal_store(uz, &vo[iv + 2 * NC]);
+            if (a.store_rho) __builtin_nontemporal_store(rho, &ro[i4]);
+        }
+
+        // STEP 3: WALE on the previous velocity (kernel_cumulant.jl:264-310)
+        float om1;
+        {
+            const float uE = VN[0][0], vE = VN[0][1], wE = VN[0][2], uW = VN[1][0], vW = VN[1][1], wW = VN[1][2];
+            const float uN = VN[2][0], vN = VN[2][1], wN = VN[2][2], uS = VN[3][0], vS = VN[3][1], wS = VN[3][2];
+            const float uT = VN[4][0], vT = VN[4][1], wT = VN[4][2], uB = VN[5][0], vB = VN[5][1], wB = VN[5][2];
+            float g11 = 0.5f * (uE - uW), g12 = 0.5f * (uN - uS), g13 = 0.5f * (uT - uB);
+            float g21 = 0.5f * (vE - vW), g22 = 0.5f * (vN - vS), g23 = 0.5f * (vT - vB);
+            float g31 = 0.5f * (wE - wW), g32 = 0.5f * (wN - wS), g33 = 0.5f * (wT - wB);
+            float q11 = g11 * g11 + g12 * g21 + g13 * g31;
+            float q12 = g11 * g12 + g12 * g22 + g13 * g32;
+            float q13 = g11 * g13 + g12 * g23 + g13 * g33;
+            float q21 = g21 * g11 + g22 * g21 + g23 * g31;
+            float q22 = g21 * g12 + g22 * g22 + g23 * g32;
+            float q23 = g21 * g13 + g22 * g23 + g23 * g33;
+            float q31 = g31 * g11 + g32 * g21 + g33 * g31;
+            float q32 = g31 * g12 + g32 * g22 + g33 * g32;
+            float q33 = g31 * g13 + g32 * g23 + g33 * g33;
+            float tr = q11 + q22 + q33;
+            float trt = tr / 3.0f;
+            float Sd11 = q11 - trt, Sd22 = q22 - trt, Sd33 = q33 - trt;
+            float Sd12 = 0.5f * (q12 + q21), Sd13 = 0.5f * (q13 + q31), Sd23 = 0.5f * (q23 + q32);
+            float S12 = 0.5f * (g12 + g21), S13 = 0.5f * (g13 + g31), S23 = 0.5f * (g23 + g32);
+            float OP1 = Sd11 * Sd11 + Sd22 * Sd22 + Sd33 * Sd33 + 2.0f * (Sd12 * Sd12 + Sd13 * Sd13 + Sd23 * Sd23);
+            float OP2 = g11 * g11 + g22 * g22 + g33 * g33 + 2.0f * (S12 * S12 + S13 * S13 + S23 * S23);
+            float nu_eddy = 0.0f;
+            if (OP1 > 1.0e-12f) {
+                float OP1_32 = OP1 * sqrtf(OP1);
+                float OP2_52 = OP2 * OP2 * sqrtf(fmaxf(OP2, 1.0e-12f));
+                float den = OP2_52 + OP1 * sqrtf(sqrtf(fmaxf(OP1, 1.0e-12f)));
+                if (den > 1.0e-12f) nu_eddy = (a.c_wale * a.c_wale) * OP1_32 / den;
+            }
+            nu_eddy = fmaxf(nu_eddy, a.nu_sgs_bg);
+            float nu_mol = (a.tau - 0.5f) / 3.0f;
+            float nu_eff = nu_mol + nu_eddy;
+            float tau_eff = 0.5f + 3.0f * nu_eff;
+            om1 = 1.0f / fmaxf(tau_eff, 0.500001f);
+        }
+        const float omb = a.omega_bulk, om3 = a.omega3, om4 = a.omega4;
+        float om4d = om4;
+        if (a.adaptive == 1) {
+            float t = om1 - 2.0f;
+            float den = a.lambda * om1 + 1.0e-10f;
+            float Av = -(t * t) / den;
+            om4d = Av > 2.0f ? 2.0f : (Av < 0.01f ? 0.01f : Av);
+        }
+
+        // STEP 4: chimera forward (kernel_cumulant.jl:330-371)
+#pragma unroll
+        for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+            for (int iy = 0; iy < 3; iy++) {
+                const int bb = 3 * iy + 9 * iz;
+                chim_fwd(A[bb], A[bb + 1], A[bb + 2], uxe);
+            }
+#pragma unroll
+        for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+            for (int ma = 0; ma < 3; ma++) {
+                const int i1 = ma + 9 * iz;
+                chim_fwd(A[i1], A[i1 + 3], A[i1 + 6], uye);
+            }
+#pragma unroll
+        for (int mb = 0; mb < 3; mb++)
+#pragma unroll
+            for (int ma = 0; ma < 3; ma++) {
+                const int i1 = ma + 3 * mb;
+                chim_fwd(A[i1], A[i1 + 9], A[i1 + 18], uze);
+            }
+
+        // STEP 5-8: cumulants, relaxation, reconstruction (kernel_cumulant.jl:376-519)
+        {
+            const float k000 = A[0];
+            const float ir = 1.0f / fmaxf(k000, 0.01f);
+            const float c200 = A[2] * ir, c020 = A[6] * ir, c002 = A[18] * ir;
+            const float c110 = A[4] * ir, c101 = A[10] * ir, c011 = A[12] * ir;
+            const float c210 = A[5] * ir, c120 = A[7] * ir, c201 = A[11] * ir;
+            const float c021 = A[15] * ir, c102 = A[19] * ir, c012 = A[21] * ir;
+            const float c111 = A[13] * ir;
+            const float c220n = A[8] * ir, c202n = A[20] * ir, c022n = A[24] * ir;
+            const float c211n = A[14] * ir, c121n = A[16] * ir, c112n = A[22] * ir;
+            const float C220 = c220n - c200 * c020 - 2.0f * c110 * c110;
+            const float C202 = c202n - c200 * c002 - 2.0f * c101 * c101;
+            const float C022 = c022n - c020 * c002 - 2.0f * c011 * c011;
+            const float C211 = c211n - c200 * c011 - 2.0f * c110 * c101;
+            const float C121 = c121n - c020 * c101 - 2.0f * c110 * c011;
+            const float C112 = c112n - c002 * c110 - 2.0f * c101 * c011;
+            const float cs2 = 0.333333333f;
+            const float Dt = c200 + c020 + c002;
+            const float Dteq = 3.0f * cs2;
+            const float Dts = (1.0f - omb) * Dt + omb * Dteq;
+            const float D1 = c200 - c020, D2 = c200 - c002;
+            const float D1s = (1.0f - om1) * D1, D2s = (1.0f - om1) * D2;
+            const float c200s = (Dts + D1s + D2s) / 3.0f;
+            const float c020s = (Dts - 2.0f * D1s + D2s) / 3.0f;
+            const float c002s = (Dts + D1s - 2.0f * D2s) / 3.0f;
+            float c110s = (1.0f - om1) * c110, c101s = (1.0f - om1) * c101, c011s = (1.0f - om1) * c011;
+            if (a.limiter == 1) {
+                float b110 = sqrtf(fmaxf(c200s * c020s, 0.0f));
+                float b101 = sqrtf(fmaxf(c200s * c002s, 0.0f));
+                float b011 = sqrtf(fmaxf(c020s * c002s, 0.0f));
+                c110s = c110s > b110 ? b110 : (c110s < -b110 ? -b110 : c110s);
+                c101s = c101s > b101 ? b101 : (c101s < -b101 ? -b101 : c101s);
+                c011s = c011s > b011 ? b011 : (c011s < -b011 ? -b011 : c011s);
+            }
+            const float c210s = (1.0f - om3) * c210, c120s = (1.0f - om3) * c120, c201s = (1.0f - om3) * c201;
+            const float c021s = (1.0f - om3) * c021, c102s = (1.0f - om3) * c102, c012s = (1.0f - om3) * c012;
+            const float c111s = (1.0f - om3) * c111;
+            const float C220s = (1.0f - om4d) * C220, C202s = (1.0f - om4d) * C202, C022s = (1.0f - om4d) * C022;
+            const float C211s = (1.0f - om4) * C211, C121s = (1.0f - om4) * C121, C112s = (1.0f - om4) * C112;
+            const float c220s = C220s + c200s * c020s + 2.0f * c110s * c110s;
+            const float c202s = C202s + c200s * c002s + 2.0f * c101s * c101s;
+            const float c022s = C022s + c020s * c002s + 2.0f * c011s * c011s;
+            const float c211s = C211s + c200s * c011s + 2.0f * c110s * c101s;
+            const float c121s = C121s + c020s * c101s + 2.0f * c110s * c011s;
+            const float c112s = C112s + c002s * c110s + 2.0f * c101s * c011s;
+            const float c221s = c200s * c021s + c020s * c201s + 4.0f * c110s * c111s + 2.0f * c101s * c120s +
+                                2.0f * c011s * c210s;
+            const float c212s = c200s * c012s + c002s * c210s + 2.0f * c110s * c102s + 4.0f * c101s * c111s +
+                                2.0f * c011s * c201s;
+            const float c122s = c020s * c102s + c002s * c120s + 2.0f * c110s * c012s + 4.0f * c011s * c111s +
+                                2.0f * c101s * c021s;
+            float c222s = c200s * C022s + c020s * C202s + c002s * C220s +
+                          4.0f * (c110s * C112s + c101s * C121s + c011s * C211s);
+            c222s += 2.0f * (c210s * c012s + c201s * c021s + c120s * c102s) + 4.0f * c111s * c111s;
+            c222s += c200s * c020s * c002s +
+                     2.0f * (c200s * c011s * c011s + c020s * c101s * c101s + c002s * c110s * c110s) +
+                     8.0f * c110s * c101s * c011s;
+            const float r = k000;
+            A[0] = r;
+            A[1] = 0.0f; A[3] = 0.0f; A[9] = 0.0f;
+            A[2] = r * c200s; A[6] = r * c020s; A[18] = r * c002s;
+            A[4] = r * c110s; A[10] = r * c101s; A[12] = r * c011s;
+            A[5] = r * c210s; A[7] = r * c120s; A[11] = r * c201s;
+            A[15] = r * c021s; A[19] = r * c102s; A[21] = r * c012s;
+            A[13] = r * c111s;
+            A[8] = r * c220s; A[20] = r * c202s; A[24] = r * c022s;
+            A[14] = r * c211s; A[16] = r * c121s; A[22] = r * c112s;
+            A[17] = r * c221s; A[23] = r * c212s; A[25] = r * c122s;
+            A[26] = r * c222s;
+        }
+
+        // STEP 9: chimera backward (kernel_cumulant.jl:525-559)
+#pragma unroll
+        for (int mb = 0; mb < 3; mb++)
+#pragma unroll
+            for (int ma = 0; ma < 3; ma++) {
+                const int i1 = ma + 3 * mb;
+                chim_bwd(A[i1], A[i1 + 9], A[i1 + 18], uze);
+            }
+#pragma unroll
+        for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+            for (int ma = 0; ma < 3; ma++) {
+                const int i1 = ma + 9 * iz;
+                chim_bwd(A[i1], A[i1 + 3], A[i1 + 6], uye);
+            }
+#pragma unroll
+        for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+            for (int iy = 0; iy < 3; iy++) {
+                const int bb = 3 * iy + 9 * iz;
+                chim_bwd(A[bb], A[bb + 1], A[bb + 2], uxe);
+            }
+
+        // positivity limiter (kernel_cumulant.jl:564-599)
+        if (a.limiter == 2) {
+            float fmin_ = A[0];
+#pragma unroll
+            for (int k = 1; k < 27; k++) fmin_ = fminf(fmin_, A[k]);
+            if (fmin_ < 0.0f) {
+                float alpha = 1.0f;
+#pragma unroll
+                for (int k = 0; k < 27; k++) {
+                    float fe = feq_auto(rho, uxe, uye, uze, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k), a.comp);
+                    float fn = A[k] - fe;
+                    if (fn < 0.0f && fe > 1.0e-15f) {
+                        float ak = fe / (fe - fn + 1.0e-15f);
+                        alpha = fminf(alpha, ak);
+                    }
+                }
+                alpha = fmaxf(alpha, 0.0f);
+#pragma unroll
+                for (int k = 0; k < 27; k++) {
+                    float fe = feq_auto(rho, uxe, uye, uze, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k), a.comp);
+                    A[k] = fe + alpha * (A[k] - fe);
                 }
             }
         }
-    }
-    const float uxe = ux + 0.5f * Fx * inv_rho;
-    const float uye = uy + 0.5f * Fy * inv_rho;
-    const float uze = uz + 0.5f * Fz * inv_rho;
-    {
-        const size_t iv = (size_t)b * 3 * NC + c;
-        __builtin_nontemporal_store(ux, &vo[iv]);
-        __builtin_nontemporal_store(uy, &vo[iv + NC]);
-        __builtin_nontemporal_store(uz, &vo[iv + 2 * NC]);
-        if (a.store_rho) __builtin_nontemporal_store(rho, &ro[i4]);
-    }
 
-    // STEP 3: WALE on the previous velocity (kernel_cumulant.jl:264-310)
-    float om1;
-    {
-        const float uE = VN[0][0], vE = VN[0][1], wE = VN[0][2], uW = VN[1][0], vW = VN[1][1], wW = VN[1][2];
-        const float uN = VN[2][0], vN = VN[2][1], wN = VN[2][2], uS = VN[3][0], vS = VN[3][1], wS = VN[3][2];
-        const float uT = VN[4][0], vT = VN[4][1], wT = VN[4][2], uB = VN[5][0], vB = VN[5][1], wB = VN[5][2];
-        float g11 = 0.5f * (uE - uW), g12 = 0.5f * (uN - uS), g13 = 0.5f * (uT - uB);
-        float g21 = 0.5f * (vE - vW), g22 = 0.5f * (vN - vS), g23 = 0.5f * (vT - vB);
-        float g31 = 0.5f * (wE - wW), g32 = 0.5f * (wN - wS), g33 = 0.5f * (wT - wB);
-        float q11 = g11 * g11 + g12 * g21 + g13 * g31;
-        float q12 = g11 * g12 + g12 * g22 + g13 * g32;
-        float q13 = g11 * g13 + g12 * g23 + g13 * g33;
-        float q21 = g21 * g11 + g22 * g21 + g23 * g31;
-        float q22 = g21 * g12 + g22 * g22 + g23 * g32;
-        float q23 = g21 * g13 + g22 * g23 + g23 * g33;
-        float q31 = g31 * g11 + g32 * g21 + g33 * g31;
-        float q32 = g31 * g12 + g32 * g22 + g33 * g32;
-        float q33 = g31 * g13 + g32 * g23 + g33 * g33;
-        float tr = q11 + q22 + q33;
-        float trt = tr / 3.0f;
-        float Sd11 = q11 - trt, Sd22 = q22 - trt, Sd33 = q33 - trt;
-        float Sd12 = 0.5f * (q12 + q21), Sd13 = 0.5f * (q13 + q31), Sd23 = 0.5f * (q23 + q32);
-        float S12 = 0.5f * (g12 + g21), S13 = 0.5f * (g13 + g31), S23 = 0.5f * (g23 + g32);
-        float OP1 = Sd11 * Sd11 + Sd22 * Sd22 + Sd33 * Sd33 + 2.0f * (Sd12 * Sd12 + Sd13 * Sd13 + Sd23 * Sd23);
-        float OP2 = g11 * g11 + g22 * g22 + g33 * g33 + 2.0f * (S12 * S12 + S13 * S13 + S23 * S23);
-        float nu_eddy = 0.0f;
-        if (OP1 > 1.0e-12f) {
-            float OP1_32 = OP1 * sqrtf(OP1);
-            float OP2_52 = OP2 * OP2 * sqrtf(fmaxf(OP2, 1.0e-12f));
-            float den = OP2_52 + OP1 * sqrtf(sqrtf(fmaxf(OP1, 1.0e-12f)));
-            if (den > 1.0e-12f) nu_eddy = (a.c_wale * a.c_wale) * OP1_32 / den;
-        }
-        nu_eddy = fmaxf(nu_eddy, a.nu_sgs_bg);
-        float nu_mol = (a.tau - 0.5f) / 3.0f;
-        float nu_eff = nu_mol + nu_eddy;
-        float tau_eff = 0.5f + 3.0f * nu_eff;
-        om1 = 1.0f / fmaxf(tau_eff, 0.500001f);
-    }
-    const float omb = a.omega_bulk, om3 = a.omega3, om4 = a.omega4;
-    float om4d = om4;
-    if (a.adaptive == 1) {
-        float t = om1 - 2.0f;
-        float den = a.lambda * om1 + 1.0e-10f;
-        float Av = -(t * t) / den;
-        om4d = Av > 2.0f ? 2.0f : (Av < 0.01f ? 0.01f : Av);
-    }
-
-    // STEP 4: chimera forward (kernel_cumulant.jl:330-371)
-#pragma unroll
-    for (int iz = 0; iz < 3; iz++)
-#pragma unroll
-        for (int iy = 0; iy < 3; iy++) {
-            const int bb = 3 * iy + 9 * iz;
-            chim_fwd(A[bb], A[bb + 1], A[bb + 2], uxe);
-        }
-#pragma unroll
-    for (int iz = 0; iz < 3; iz++)
-#pragma unroll
-        for (int ma = 0; ma < 3; ma++) {
-            const int i1 = ma + 9 * iz;
-            chim_fwd(A[i1], A[i1 + 3], A[i1 + 6], uye);
-        }
-#pragma unroll
-    for (int mb = 0; mb < 3; mb++)
-#pragma unroll
-        for (int ma = 0; ma < 3; ma++) {
-            const int i1 = ma + 3 * mb;
-            chim_fwd(A[i1], A[i1 + 9], A[i1 + 18], uze);
-        }
-
-    // STEP 5-8: cumulants, relaxation, reconstruction (kernel_cumulant.jl:376-519)
-    {
-        const float k000 = A[0];
-        const float ir = 1.0f / fmaxf(k000, 0.01f);
-        const float c200 = A[2] * ir, c020 = A[6] * ir, c002 = A[18] * ir;
-        const float c110 = A[4] * ir, c101 = A[10] * ir, c011 = A[12] * ir;
-        const float c210 = A[5] * ir, c120 = A[7] * ir, c201 = A[11] * ir;
-        const float c021 = A[15] * ir, c102 = A[19] * ir, c012 = A[21] * ir;
-        const float c111 = A[13] * ir;
-        const float c220n = A[8] * ir, c202n = A[20] * ir, c022n = A[24] * ir;
-        const float c211n = A[14] * ir, c121n = A[16] * ir, c112n = A[22] * ir;
-        const float C220 = c220n - c200 * c020 - 2.0f * c110 * c110;
-        const float C202 = c202n - c200 * c002 - 2.0f * c101 * c101;
-        const float C022 = c022n - c020 * c002 - 2.0f * c011 * c011;
-        const float C211 = c211n - c200 * c011 - 2.0f * c110 * c101;
-        const float C121 = c121n - c020 * c101 - 2.0f * c110 * c011;
-        const float C112 = c112n - c002 * c110 - 2.0f * c101 * c011;
-        const float cs2 = 0.333333333f;
-        const float Dt = c200 + c020 + c002;
-        const float Dteq = 3.0f * cs2;
-        const float Dts = (1.0f - omb) * Dt + omb * Dteq;
-        const float D1 = c200 - c020, D2 = c200 - c002;
-        const float D1s = (1.0f - om1) * D1, D2s = (1.0f - om1) * D2;
-        const float c200s = (Dts + D1s + D2s) / 3.0f;
-        const float c020s = (Dts - 2.0f * D1s + D2s) / 3.0f;
-        const float c002s = (Dts + D1s - 2.0f * D2s) / 3.0f;
-        float c110s = (1.0f - om1) * c110, c101s = (1.0f - om1) * c101, c011s = (1.0f - om1) * c011;
-        if (a.limiter == 1) {
-            float b110 = sqrtf(fmaxf(c200s * c020s, 0.0f));
-            float b101 = sqrtf(fmaxf(c200s * c002s, 0.0f));
-            float b011 = sqrtf(fmaxf(c020s * c002s, 0.0f));
-            c110s = c110s > b110 ? b110 : (c110s < -b110 ? -b110 : c110s);
-            c101s = c101s > b101 ? b101 : (c101s < -b101 ? -b101 : c101s);
-            c011s = c011s > b011 ? b011 : (c011s < -b011 ? -b011 : c011s);
-        }
-        const float c210s = (1.0f - om3) * c210, c120s = (1.0f - om3) * c120, c201s = (1.0f - om3) * c201;
-        const float c021s = (1.0f - om3) * c021, c102s = (1.0f - om3) * c102, c012s = (1.0f - om3) * c012;
-        const float c111s = (1.0f - om3) * c111;
-        const float C220s = (1.0f - om4d) * C220, C202s = (1.0f - om4d) * C202, C022s = (1.0f - om4d) * C022;
-        const float C211s = (1.0f - om4) * C211, C121s = (1.0f - om4) * C121, C112s = (1.0f - om4) * C112;
-        const float c220s = C220s + c200s * c020s + 2.0f * c110s * c110s;
-        const float c202s = C202s + c200s * c002s + 2.0f * c101s * c101s;
-        const float c022s = C022s + c020s * c002s + 2.0f * c011s * c011s;
-        const float c211s = C211s + c200s * c011s + 2.0f * c110s * c101s;
-        const float c121s = C121s + c020s * c101s + 2.0f * c110s * c011s;
-        const float c112s = C112s + c002s * c110s + 2.0f * c101s * c011s;
-        const float c221s = c200s * c021s + c020s * c201s + 4.0f * c110s * c111s + 2.0f * c101s * c120s +
-                            2.0f * c011s * c210s;
-        const float c212s = c200s * c012s + c002s * c210s + 2.0f * c110s * c102s + 4.0f * c101s * c111s +
-                            2.0f * c011s * c201s;
-        const float c122s = c020s * c102s + c002s * c120s + 2.0f * c110s * c012s + 4.0f * c011s * c111s +
-                            2.0f * c101s * c021s;
-        float c222s = c200s * C022s + c020s * C202s + c002s * C220s +
-                      4.0f * (c110s * C112s + c101s * C121s + c011s * C211s);
-        c222s += 2.0f * (c210s * c012s + c201s * c021s + c120s * c102s) + 4.0f * c111s * c111s;
-        c222s += c200s * c020s * c002s +
-                 2.0f * (c200s * c011s * c011s + c020s * c101s * c101s + c002s * c110s * c110s) +
-                 8.0f * c110s * c101s * c011s;
-        const float r = k000;
-        A[0] = r;
-        A[1] = 0.0f; A[3] = 0.0f; A[9] = 0.0f;
-        A[2] = r * c200s; A[6] = r * c020s; A[18] = r * c002s;
-        A[4] = r * c110s; A[10] = r * c101s; A[12] = r * c011s;
-        A[5] = r * c210s; A[7] = r * c120s; A[11] = r * c201s;
-        A[15] = r * c021s; A[19] = r * c102s; A[21] = r * c012s;
-        A[13] = r * c111s;
-        A[8] = r * c220s; A[20] = r * c202s; A[24] = r * c022s;
-        A[14] = r * c211s; A[16] = r * c121s; A[22] = r * c112s;
-        A[17] = r * c221s; A[23] = r * c212s; A[25] = r * c122s;
-        A[26] = r * c222s;
-    }
-
-    // STEP 9: chimera backward (kernel_cumulant.jl:525-559)
-#pragma unroll
-    for (int mb = 0; mb < 3; mb++)
-#pragma unroll
-        for (int ma = 0; ma < 3; ma++) {
-            const int i1 = ma + 3 * mb;
-            chim_bwd(A[i1], A[i1 + 9], A[i1 + 18], uze);
-        }
-#pragma unroll
-    for (int iz = 0; iz < 3; iz++)
-#pragma unroll
-        for (int ma = 0; ma < 3; ma++) {
-            const int i1 = ma + 9 * iz;
-            chim_bwd(A[i1], A[i1 + 3], A[i1 + 6], uye);
-        }
-#pragma unroll
-    for (int iz = 0; iz < 3; iz++)
-#pragma unroll
-        for (int iy = 0; iy < 3; iy++) {
-            const int bb = 3 * iy + 9 * iz;
-            chim_bwd(A[bb], A[bb + 1], A[bb + 2], uxe);
-        }
-
-    // positivity limiter (kernel_cumulant.jl:564-599)
-    if (a.limiter == 2) {
-        float fmin_ = A[0];
-#pragma unroll
-        for (int k = 1; k < 27; k++) fmin_ = fminf(fmin_, A[k]);
-        if (fmin_ < 0.0f) {
-            float alpha = 1.0f;
+        // STEP 10: Guo forcing + store (kernel_cumulant.jl:604-623).  With F = 0
+        // the forcing term is a signed zero and leaves every population unchanged.
+        if (Fx != 0.0f || Fy != 0.0f || Fz != 0.0f) {
+            const float pre = 1.0f - 0.5f * om1;
 #pragma unroll
             for (int k = 0; k < 27; k++) {
-                float fe = feq_auto(rho, uxe, uye, uze, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k), a.comp);
-                float fn = A[k] - fe;
-                if (fn < 0.0f && fe > 1.0e-15f) {
-                    float ak = fe / (fe - fn + 1.0e-15f);
-                    alpha = fminf(alpha, ak);
-                }
+                const float cxf = (float)CX(k), cyf = (float)CY(k), czf = (float)CZ(k);
+                float cu = cxf * uxe + cyf * uye + czf * uze;
+                float ft = WK(k) * 3.0f *
+                           ((cxf - ux + 3.0f * cu * cxf) * Fx + (cyf - uy + 3.0f * cu * cyf) * Fy +
+                            (czf - uz + 3.0f * cu * czf) * Fz);
+                if constexpr (STAGE)
+                    A[k] = A[k] + pre * ft;
+                else
+                    Codec<ST>::stream(fo, fidx(b, k, c), A[k] + pre * ft, WK(k));
             }
-            alpha = fmaxf(alpha, 0.0f);
+        } else if constexpr (!STAGE) {
 #pragma unroll
-            for (int k = 0; k < 27; k++) {
-                float fe = feq_auto(rho, uxe, uye, uze, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k), a.comp);
-                A[k] = fe + alpha * (A[k] - fe);
-            }
+            for (int k = 0; k < 27; k++) Codec<ST>::stream(fo, fidx(b, k, c), A[k], WK(k));
         }
-    }
-
-    // STEP 10: Guo forcing + store (kernel_cumulant.jl:604-623).  With F = 0
-    // the forcing term is a signed zero and leaves every population unchanged.
-    if (Fx != 0.0f || Fy != 0.0f || Fz != 0.0f) {
-        const float pre = 1.0f - 0.5f * om1;
-#pragma unroll
-        for (int k = 0; k < 27; k++) {
-            const float cxf = (float)CX(k), cyf = (float)CY(k), czf = (float)CZ(k);
-            float cu = cxf * uxe + cyf * uye + czf * uze;
-            float ft = WK(k) * 3.0f *
-                       ((cxf - ux + 3.0f * cu * cxf) * Fx + (cyf - uy + 3.0f * cu * cyf) * Fy +
-                        (czf - uz + 3.0f * cu * czf) * Fz);
-            Codec<ST>::stream(fo, fidx(b, k, c), A[k] + pre * ft, WK(k));
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 27; k++) Codec<ST>::stream(fo, fidx(b, k, c), A[k], WK(k));
-    }
+    }  // !is_obs
+    if constexpr (STAGE) stream_tile<ST>(fo, b, c, A, sst + wv * (27 * 64));
 }
 
 inline unsigned padded_groups(int nb) { return ((unsigned)nb * 2u + 7u) & ~7u; }
