@@ -1041,6 +1041,9 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
     __shared__ OffT ptab[108];
     __shared__ uint64_t smask[2];
     __shared__ float sfix[BND ? 27 * 256 : 1];
+    // interior blocks: outputs leave through stream_tile, as in the collide
+    constexpr bool STAGE = !BND;
+    __shared__ __attribute__((aligned(16))) Stored<ST> sst[STAGE ? 4 * 27 * 64 : 8];
     Work wk;
     int half;
     if (!schedule(work, n_tiles, sched, wk, half)) return;
@@ -1112,31 +1115,39 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
                 float gw = geq(a.T_wall, 0.0f, 0.0f, 0.0f, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k));
                 go = 2.0f * gw - gopp;
             }
-            Codec<ST>::stream(a.g_out, fidx(b, k, c), go, WK(k));
+            if constexpr (STAGE)
+                G[k] = go;
+            else
+                Codec<ST>::stream(a.g_out, fidx(b, k, c), go, WK(k));
         }
         if (a.store_T) a.T_out[i4] = a.adiabatic == 1 ? Tsum : a.T_wall;
-        return;
-    }
-    float Tl = fmaxf(Tsum, 0.01f);
-    const float sp = a.sponge[i4];
-    if (sp > 0.0f) {
-        Tl = Tl * (1.0f - sp) + a.T_inlet * sp;
-        ux = ux * (1.0f - sp) + a.u_inlet * sp;
-        uy = uy * (1.0f - sp);
-        uz = uz * (1.0f - sp);
-    }
-    if (a.store_T) a.T_out[i4] = Tl;
-    const float omg = 1.0f / fmaxf(a.tau_g, 0.500001f);
-#pragma unroll
-    for (int k = 0; k < 27; k++) {
-        float gs = G[k];
-        if (sp > 0.0f && a.sponge_blend == 1) {
-            float gt = geq(a.T_inlet, a.u_inlet, 0.0f, 0.0f, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k));
-            gs = gs * (1.0f - sp) + gt * sp;
+        if constexpr (!STAGE) return;
+    } else {
+        float Tl = fmaxf(Tsum, 0.01f);
+        const float sp = a.sponge[i4];
+        if (sp > 0.0f) {
+            Tl = Tl * (1.0f - sp) + a.T_inlet * sp;
+            ux = ux * (1.0f - sp) + a.u_inlet * sp;
+            uy = uy * (1.0f - sp);
+            uz = uz * (1.0f - sp);
         }
-        float ge = geq(Tl, ux, uy, uz, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k));
-        Codec<ST>::stream(a.g_out, fidx(b, k, c), gs - omg * (gs - ge), WK(k));
-    }
+        if (a.store_T) a.T_out[i4] = Tl;
+        const float omg = 1.0f / fmaxf(a.tau_g, 0.500001f);
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+            float gs = G[k];
+            if (sp > 0.0f && a.sponge_blend == 1) {
+                float gt = geq(a.T_inlet, a.u_inlet, 0.0f, 0.0f, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k));
+                gs = gs * (1.0f - sp) + gt * sp;
+            }
+            float ge = geq(Tl, ux, uy, uz, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k));
+            if constexpr (STAGE)
+                G[k] = gs - omg * (gs - ge);
+            else
+                Codec<ST>::stream(a.g_out, fidx(b, k, c), gs - omg * (gs - ge), WK(k));
+        }
+    }  // !is_obs
+    if constexpr (STAGE) stream_tile<ST>(a.g_out, b, c, G, sst + wv * (27 * 64));
 }
 
 // Interior blocks (every neighbour present, ghosts included) run the variant
