@@ -337,32 +337,38 @@ struct Codec<2> {
 
 // Staged stores of a wave's stream-collide outputs.  A wave is one z-plane
 // (64 consecutive cells) of its block, so row k of its output is 64
-// consecutive slots of f[b][k][*].  Each lane writes its cell's 27 stored
-// values into the wave's [27][64] tile in LDS; then each lane stores 16
-// bytes of one row, so the outputs leave in 7 (FP32) or 4 (16-bit) vector
-// stores per wave instead of 27 scalar ones.  The LDS operations of one wave
-// complete in order, so the two phases need no barrier.
+// consecutive slots of f[b][k][*].  Each lane writes its cell's stored values
+// for ROWS directions into the wave's [ROWS][64] tile in LDS; then each lane
+// stores 16 bytes of one row.  With ROWS = 27 the outputs leave in 7 (FP32)
+// or 4 (16-bit) vector stores per wave instead of 27 scalar ones; ROWS = 14
+// halves the tile for 8 stores in two phases.  The LDS operations of one
+// wave complete in order, so the phases need no barrier.
 template <int ST>
 using Stored = std::conditional_t<ST == 0, uint32_t, uint16_t>;
 
-template <int ST>
+template <int ST, int ROWS>
 __device__ __forceinline__ void stream_tile(void* fo, int b, int c, const float (&A)[27], Stored<ST>* tile) {
     constexpr int EPL = 16 / (int)sizeof(Stored<ST>);  // values per 16-byte store
     constexpr int LPR = 64 / EPL;                      // lanes per row of 64 values
     const int lane = c & 63;
-#pragma unroll
-    for (int k = 0; k < 27; k++) tile[k * 64 + lane] = Codec<ST>::bits(A[k], WK(k));
-    __builtin_amdgcn_wave_barrier();
     const int sub = lane % LPR;
     const int c0 = c & ~63;
 #pragma unroll
-    for (int i = 0; i < (27 + EPL - 1) / EPL; i++) {
-        const int k = EPL * i + lane / LPR;
-        if (k < 27) {
-            const v4u v = *(const v4u*)(tile + k * 64 + sub * EPL);
-            const size_t slot = fidx(b, k, c0) + (size_t)(sub * EPL);
-            __builtin_nontemporal_store(v, (GV4U*)((GC*)fo + slot * sizeof(Stored<ST>)));
+    for (int k0 = 0; k0 < 27; k0 += ROWS) {
+        const int nr = 27 - k0 < ROWS ? 27 - k0 : ROWS;
+#pragma unroll
+        for (int r = 0; r < nr; r++) tile[r * 64 + lane] = Codec<ST>::bits(A[k0 + r], WK(k0 + r));
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int i = 0; i < (nr + EPL - 1) / EPL; i++) {
+            const int r = EPL * i + lane / LPR;
+            if (r < nr) {
+                const v4u v = *(const v4u*)(tile + r * 64 + sub * EPL);
+                const size_t slot = fidx(b, k0 + r, c0) + (size_t)(sub * EPL);
+                __builtin_nontemporal_store(v, (GV4U*)((GC*)fo + slot * sizeof(Stored<ST>)));
+            }
         }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -552,6 +558,49 @@ __device__ __forceinline__ const GF* gaddr(const void* base, OffT off) {
     return (const GF*)((const GC*)base + off);
 }
 
+// Velocity tile of a work-group (half a block, 4 z-planes) for the WALE
+// stencil: per component, planes zz = -1..4 (the 4 own planes and the two
+// z-face halos) of 10 x 10 cells (x, y = -1..8; of the halo ring only the
+// faces are filled and read).  Every lane loads its own cell; wave 0 loads the
+// -z face plane, wave 1 the +z plane, wave 2 the two x faces and wave 3 the
+// two y faces, each value through the stencil table (the neighbour block's
+// cell, or the adjacent own cell where the neighbour is outside the level).
+// 6 loads per lane instead of the 18 of a per-lane stencil.
+constexpr int VT_SIZE = 6 * 100;
+__device__ __forceinline__ int vt_index(int d, int zz, int yy, int xx) {
+    return d * VT_SIZE + (zz + 1) * 100 + (yy + 1) * 10 + (xx + 1);
+}
+template <class OffT>
+__device__ __forceinline__ void vel_tile(const float* vel, const OffT* vtab, int half, int c, float* vt) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int x = c & 7, y = (c >> 3) & 7;
+    const GF* p = gaddr(vel, vtab[wv * 8 + 6] + (OffT)(4 * c));
+    const float o0 = p[0], o1 = p[NC], o2 = p[2 * NC];
+    int n, ce, hz, hy, hx;  // face, own edge cell, halo position
+    if (wv == 0) {
+        n = 5, ce = (half << 8) + lane, hz = -1, hy = lane >> 3, hx = lane & 7;
+    } else if (wv == 1) {
+        n = 4, ce = (half << 8) + 192 + lane, hz = 4, hy = lane >> 3, hx = lane & 7;
+    } else {
+        const int side = lane >> 5, t = (lane >> 2) & 7, zl = lane & 3;
+        if (wv == 2)
+            n = side ? 0 : 1, ce = (half << 8) + zl * 64 + 8 * t + (side ? 7 : 0), hz = zl, hy = t, hx = side ? 8 : -1;
+        else
+            n = side ? 2 : 3, ce = (half << 8) + zl * 64 + 8 * (side ? 7 : 0) + t, hz = zl, hy = side ? 8 : -1, hx = t;
+    }
+    const int dx = n == 0 ? 1 : (n == 1 ? -1 : 0);
+    const int dy = n == 2 ? 1 : (n == 3 ? -1 : 0);
+    const int dz = n == 4 ? 1 : (n == 5 ? -1 : 0);
+    const GF* q = gaddr(vel, vtab[((ce >> 6) & 3) * 8 + n] + (OffT)(4 * (ce + dx + 8 * dy + 64 * dz)));
+    const float h0 = q[0], h1 = q[NC], h2 = q[2 * NC];
+    vt[vt_index(0, wv, y, x)] = o0;
+    vt[vt_index(1, wv, y, x)] = o1;
+    vt[vt_index(2, wv, y, x)] = o2;
+    vt[vt_index(0, hz, hy, hx)] = h0;
+    vt[vt_index(1, hz, hy, hx)] = h1;
+    vt[vt_index(2, hz, hy, hx)] = h2;
+}
+
 // stream_collide_cumulant! (src/collision/kernel_cumulant.jl:29-627), with the
 // A/B buffers chosen by the caller and no f_post_collision copy (Bouzidi
 // reads f_out before it corrects it, see bouzidi_gather below).
@@ -578,7 +627,16 @@ collide_kernel(const LevelArgs* __restrict__ las,
     // interior blocks: outputs leave through stream_tile (the boundary
     // kernel's cascade buffer already takes the LDS budget)
     constexpr bool STAGE = !BND;
-    __shared__ __attribute__((aligned(16))) Stored<ST> sst[STAGE ? 4 * 27 * 64 : 8];
+    // FP32 interior blocks: the work-group's previous velocity with its face
+    // halos goes through LDS for the WALE stencil (vel_tile), and the output
+    // tile is halved so that both fit 5 work-groups per CU.  Measured on the
+    // 52.5M-cell cube case, finest collide: FP32 -1.9 % with the velocity
+    // tile; FP16 +1.5 % (the work-group barrier), so 16-bit storage loads the
+    // stencil per lane.
+    constexpr bool VTILE = STAGE && ST == 0;
+    constexpr int ROWS = ST == 0 ? 14 : 27;
+    __shared__ __attribute__((aligned(16))) Stored<ST> sst[STAGE ? 4 * ROWS * 64 : 8];
+    __shared__ float vt[VTILE ? 3 * VT_SIZE : 1];
     Work wk;
     int half;
     if (!schedule(work, n_tiles, sched, wk, half)) return;
@@ -619,7 +677,18 @@ collide_kernel(const LevelArgs* __restrict__ las,
     }
     // WALE stencil of the previous velocity, loaded together with the populations.
     float VN[6][3];
-    {
+    if constexpr (VTILE) {
+        vel_tile<OffT>(a.vel_in, vtab, half, c, vt);
+        __syncthreads();
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            const int dx = n == 0 ? 1 : (n == 1 ? -1 : 0);
+            const int dy = n == 2 ? 1 : (n == 3 ? -1 : 0);
+            const int dz = n == 4 ? 1 : (n == 5 ? -1 : 0);
+#pragma unroll
+            for (int d = 0; d < 3; d++) VN[n][d] = vt[vt_index(d, wv + dz, y + dy, x + dx)];
+        }
+    } else {
         const OffT own = vtab[wv * 8 + 6];
 #pragma unroll
         for (int n = 0; n < 6; n++) {
@@ -985,7 +1054,7 @@ collide_kernel(const LevelArgs* __restrict__ las,
             for (int k = 0; k < 27; k++) Codec<ST>::stream(fo, fidx(b, k, c), A[k], WK(k));
         }
     }  // !is_obs
-    if constexpr (STAGE) stream_tile<ST>(fo, b, c, A, sst + wv * (27 * 64));
+    if constexpr (STAGE) stream_tile<ST, ROWS>(fo, b, c, A, sst + wv * (ROWS * 64));
 }
 
 inline unsigned padded_groups(int nb) { return ((unsigned)nb * 2u + 7u) & ~7u; }
@@ -1147,7 +1216,7 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
                 Codec<ST>::stream(a.g_out, fidx(b, k, c), gs - omg * (gs - ge), WK(k));
         }
     }  // !is_obs
-    if constexpr (STAGE) stream_tile<ST>(a.g_out, b, c, G, sst + wv * (27 * 64));
+    if constexpr (STAGE) stream_tile<ST, 27>(a.g_out, b, c, G, sst + wv * (27 * 64));
 }
 
 // Interior blocks (every neighbour present, ghosts included) run the variant
