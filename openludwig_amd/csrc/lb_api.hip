@@ -85,7 +85,9 @@ struct Level {
     int32_t ngh = 0;
     int64_t ngent = 0;
     int32_t* gent = nullptr;
+    int32_t* gpar = nullptr;  // per ghost entry: its 8 parent cells, (block << 9) | cell, or -1
     int32_t pd[3] = {0, 0, 0};
+    std::vector<int32_t> grid_host;  // host copy of grid, for the child level's ghost parents
     // Bouzidi link list
     int64_t nlinks = 0;
     int64_t* ldst = nullptr;
@@ -548,6 +550,7 @@ int upload_level_args(lb_ctx* c, int li) {
         A.ptau_g = P.tau_g;
     }
     A.gent = L.gent;
+    A.gpar = L.gpar;
     A.ngent = L.ngent;
     A.links.n = L.nlinks;
     A.links.dst = L.ldst;
@@ -877,6 +880,39 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     }
     L.ngh = ngh;
     L.ngent = (int64_t)gent.size() / 2;
+    // The 8 parent cells of every ghost entry, as the interpolation of
+    // physics_interpolation.jl:4-127 addresses them: the fine cell centre at
+    // (g - 0.5) / 2 in parent cells, its lower corner clamped to the domain,
+    // each corner's parent block looked up in the parent grid (-1 where the
+    // parent level holds no block).  Static, so the kernel reads them instead
+    // of chasing the block origin and the parent grid per entry.
+    std::vector<int32_t> gpar((size_t)L.ngent * 8, -1);
+    if (L.ngent > 0) {
+        const Level& P = c->levels[(size_t)L.id - 2];
+        const int pdx = P.pd[0], pdy = P.pd[1], pdz = P.pd[2];
+        for (int64_t e = 0; e < L.ngent; e++) {
+            const int gb = gent[(size_t)e * 2] >> 9, cl = gent[(size_t)e * 2] & 511;
+            const int fg[3] = {borg[(size_t)gb * 3 + 0] + (cl & 7) + 1, borg[(size_t)gb * 3 + 1] + ((cl >> 3) & 7) + 1,
+                               borg[(size_t)gb * 3 + 2] + (cl >> 6) + 1};
+            int lo[3], hi[3];
+            for (int ax = 0; ax < 3; ax++) {
+                const float pc = ((float)fg[ax] - 0.5f) * 0.5f;
+                lo[ax] = (int)std::floor(pc);
+                hi[ax] = lo[ax] + 1;
+                lo[ax] = std::max(1, lo[ax]);
+            }
+            for (int q = 0; q < 8; q++) {
+                const int pgx = (q & 1) ? hi[0] : lo[0], pgy = (q & 2) ? hi[1] : lo[1], pgz = (q & 4) ? hi[2] : lo[2];
+                const int pbx = (pgx - 1) / 8 + 1, pby = (pgy - 1) / 8 + 1, pbz = (pgz - 1) / 8 + 1;
+                if (pbx < 1 || pbx > pdx || pby < 1 || pby > pdy || pbz < 1 || pbz > pdz) continue;
+                const int32_t pb = P.grid_host[(size_t)(pbx - 1) + (size_t)pdx * ((size_t)(pby - 1) + (size_t)pdy * (pbz - 1))];
+                if (pb < 0) continue;
+                if (pb >= (1 << 22)) return fail(c, LB_EINVAL, "level %d: parent block index %d too large", L.id, pb);
+                const int cc = ((pgx - 1) % 8) + 8 * ((pgy - 1) % 8) + 64 * ((pgz - 1) % 8);
+                gpar[(size_t)e * 8 + q] = (pb << 9) | cc;
+            }
+        }
+    }
     const size_t n4 = (size_t)nb * NC, n5 = ((size_t)nb + ngh) * NC * Q;
     L.wide = (n5 + 512) * pop_bytes(c->cfg.storage) >= ((size_t)1 << 32);
     L.wide_g = L.wide;  // thermal populations use the same storage
@@ -903,12 +939,14 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     RC(dalloc(c, &L.borg, borg.size()));
     RC(dalloc(c, &L.order, (size_t)nb));
     RC(dalloc(c, &L.gent, gent.size()));
+    RC(dalloc(c, &L.gpar, gpar.size()));
     const size_t ng = (size_t)d->ptr_dim_x * d->ptr_dim_y * d->ptr_dim_z;
     RC(dalloc(c, &L.grid, ng));
     L.pd[0] = d->ptr_dim_x;
     L.pd[1] = d->ptr_dim_y;
     L.pd[2] = d->ptr_dim_z;
     if (!gent.empty()) RC(h2d(c, L.gent, gent.data(), gent.size()));
+    if (!gpar.empty()) RC(h2d(c, L.gpar, gpar.data(), gpar.size()));
     if (nb > 0) {
         RC(h2d(c, L.nbr, nbr.data(), nbr.size()));
         RC(h2d(c, L.borg, borg.data(), borg.size()));
@@ -981,6 +1019,7 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
             grid[i] = v - 1;
         }
         RC(h2d(c, L.grid, grid.data(), ng));
+        L.grid_host = std::move(grid);
     }
     RC(build_links(c, L, d));
     RC(chunk_list(c, L.id - 1, L.ngent, &L.wgh, &L.n_gchunks));

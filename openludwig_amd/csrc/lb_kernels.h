@@ -91,6 +91,8 @@ struct LevelArgs {
     float ptau, ptau_g;
     // ghost entries: [n][2] = (ghost block << 9 | cell, 27-bit direction mask)
     const int32_t* gent;
+    // per ghost entry: its 8 parent cells (parent block << 9 | cell, or -1)
+    const int32_t* gpar;
     int64_t ngent;
     BouzidiLinks links;
 };

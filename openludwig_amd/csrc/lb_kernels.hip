@@ -97,16 +97,14 @@ struct GhostArgs {
     void* f;   // input half of substep j (ghost blocks at index >= nb)
     void* f2;  // input half of substep j + 1
     const int32_t* entries;
+    const int32_t* parents;  // [n][8] parent cells of each entry (LevelArgs::gpar)
     int64_t n;
-    const int32_t* borg;
     const void* a_new;
     const float* s_new;
     const float* u_new;
     const void* a_old;
     const float* s_old;
     const float* u_old;
-    const int32_t* pgrid;
-    int32_t pdx, pdy, pdz;
     float tau_fine, tau_coarse;
     int32_t temporal, comp, thermal;
 };
@@ -234,17 +232,13 @@ __device__ __forceinline__ GhostArgs ghost_view(const LevelArgs& L, const CaseSt
     g.f2 = thermal ? (void*)L.g[1 - par] : L.f[1 - par];
     g.entries = L.gent;
     g.n = L.ngent;
-    g.borg = L.borg;
+    g.parents = L.gpar;
     g.a_new = thermal ? (const void*)L.pg[1 - ppar] : L.pf[1 - ppar];
     g.s_new = thermal ? L.pT[1 - ppar] : L.prho[1 - ppar];
     g.u_new = L.pvel[1 - ppar];
     g.a_old = thermal ? (const void*)L.pg[ppar] : L.pf[ppar];
     g.s_old = thermal ? L.pT[ppar] : L.prho[ppar];
     g.u_old = L.pvel[ppar];
-    g.pgrid = L.pgrid;
-    g.pdx = L.pdx;
-    g.pdy = L.pdy;
-    g.pdz = L.pdz;
     g.tau_fine = thermal ? L.tau_g : L.tau;
     g.tau_coarse = thermal ? L.ptau_g : L.ptau;
     g.temporal = S.temporal;
@@ -1336,18 +1330,23 @@ __global__ void __launch_bounds__(256) ghost_kernel(const LevelArgs* __restrict_
     const int e0 = a.entries[2 * i];
     const uint32_t mask = (uint32_t)a.entries[2 * i + 1];
     const int gb = e0 >> 9, c = e0 & 511;
-    const int fgx = a.borg[gb * 3 + 0] + (c & 7) + 1;
-    const int fgy = a.borg[gb * 3 + 1] + ((c >> 3) & 7) + 1;
-    const int fgz = a.borg[gb * 3 + 2] + (c >> 6) + 1;
-    const float pxc = ((float)fgx - 0.5f) * 0.5f;
-    const float pyc = ((float)fgy - 0.5f) * 0.5f;
-    const float pzc = ((float)fgz - 0.5f) * 0.5f;
-    int px0 = (int)floorf(pxc), py0 = (int)floorf(pyc), pz0 = (int)floorf(pzc);
-    const int px1 = px0 + 1, py1 = py0 + 1, pz1 = pz0 + 1;
-    const float wx = pxc - (float)px0, wy = pyc - (float)py0, wz = pzc - (float)pz0;
-    px0 = max(1, px0);
-    py0 = max(1, py0);
-    pz0 = max(1, pz0);
+    // The fine cell centre lies at (g - 0.5) / 2 in parent cells, g = block
+    // origin (a multiple of 8) + local index + 1, so its fractional part is
+    // 0.25 for an even local index and 0.75 for an odd one (exactly what
+    // (g - 0.5) * 0.5 - floor(...) gives in FP32).  The 8 parent cells come
+    // precomputed with the entry.
+    const float wx = (c & 1) ? 0.75f : 0.25f;
+    const float wy = ((c >> 3) & 1) ? 0.75f : 0.25f;
+    const float wz = ((c >> 6) & 1) ? 0.75f : 0.25f;
+    int par[8];
+    {
+        typedef int32_t v4i __attribute__((ext_vector_type(4)));
+        const __attribute__((address_space(1))) v4i* pp =
+            (const __attribute__((address_space(1))) v4i*)(a.parents + 8 * i);
+        const v4i p0 = pp[0], p1 = pp[1];
+        par[0] = p0.x, par[1] = p0.y, par[2] = p0.z, par[3] = p0.w;
+        par[4] = p1.x, par[5] = p1.y, par[6] = p1.z, par[7] = p1.w;
+    }
     // temporal blend with tw = 0 (substep j) and tw = 0.5 (substep j + 1)
     const bool use_old0 = a.temporal == 1 && 0.0f < 0.99f;
     const bool use_old1 = a.temporal == 1 && 0.5f < 0.99f;
@@ -1357,21 +1356,9 @@ __global__ void __launch_bounds__(256) ghost_kernel(const LevelArgs* __restrict_
     float v0[8][4], v1[8][4];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-        const int pgx = (q & 1) ? px1 : px0;
-        const int pgy = (q & 2) ? py1 : py0;
-        const int pgz = (q & 4) ? pz1 : pz0;
-        const int pbx = (pgx - 1) / BS + 1, pby = (pgy - 1) / BS + 1, pbz = (pgz - 1) / BS + 1;
-        ok[q] = false;
-        cb[q] = 0;
-        cc[q] = 0;
-        if (pbx >= 1 && pbx <= a.pdx && pby >= 1 && pby <= a.pdy && pbz >= 1 && pbz <= a.pdz) {
-            const int pb = a.pgrid[(pbx - 1) + a.pdx * ((pby - 1) + a.pdy * (pbz - 1))];
-            if (pb >= 0) {
-                ok[q] = true;
-                cb[q] = pb;
-                cc[q] = ((pgx - 1) % BS) + BS * ((pgy - 1) % BS) + BS * BS * ((pgz - 1) % BS);
-            }
-        }
+        ok[q] = par[q] >= 0;
+        cb[q] = ok[q] ? (par[q] >> 9) : 0;
+        cc[q] = ok[q] ? (par[q] & 511) : 0;
         if (ok[q]) {
             const size_t i4 = (size_t)cb[q] * NC + cc[q];
             const size_t iv = (size_t)cb[q] * 3 * NC + cc[q];
