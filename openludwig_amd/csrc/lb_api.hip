@@ -384,6 +384,24 @@ int build_links(lb_ctx* c, Level& L, const lb_level_desc* d) {
     L.nlinks = (int64_t)dst.size();
     if (L.nlinks == 0) return LB_OK;
     size_t n = (size_t)L.nlinks;
+    // Link order does not change results (every link is gathered before any
+    // is scattered, and each destination slot has one link).  Sorted by
+    // destination, a wave's links share (block, direction) rows, so the
+    // scatter's stores and the gather's loads of the same row coalesce.
+    {
+        std::vector<size_t> perm(n);
+        for (size_t i = 0; i < n; i++) perm[i] = i;
+        std::sort(perm.begin(), perm.end(), [&](size_t u, size_t v) { return dst[u] < dst[v]; });
+        auto apply = [&](auto& vec) {
+            auto tmp = vec;
+            for (size_t i = 0; i < n; i++) vec[i] = tmp[perm[i]];
+        };
+        apply(dst);
+        apply(s1);
+        apply(s2);
+        apply(ca);
+        apply(cb);
+    }
     RC(dalloc(c, &L.ldst, n));
     RC(dalloc(c, &L.ls1, n));
     RC(dalloc(c, &L.ls2, n));
