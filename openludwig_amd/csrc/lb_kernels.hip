@@ -603,11 +603,12 @@ __device__ __forceinline__ void vel_tile(const float* vel, const OffT* vtab, int
 // which stays in cache, and lets more waves hide the pull's memory latency.
 // Boundary blocks (some neighbour outside the domain) need the 27 KB cascade
 // buffer in LDS, which limits them to 5 work-groups per CU; they run at 5.
-// Interior blocks need no cascade buffer; W = 5 or 6 is a launch option
+// Interior blocks need no cascade buffer (their LDS holds the output tile and,
+// in FP32, the velocity tile: 14-23 KB); W = 5 or 6 is a launch option
 // (picked by the autotuner).  The two compile to the same floating-point
 // operations (tested bit-identical); a 4-wave build contracts differently and
-// is not offered.  Measured on the 52.5M-cell cube case: FP16 best at 5,
-// FP32 equal at 4 to 6.
+// is not offered.  Measured on the 52.5M-cell cube case with the staged
+// stores: 5 waves is best (6 waves spills more: FP32 +5 %, FP16 +7 %).
 template <int ST, class OffT, bool BND, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 collide_kernel(const LevelArgs* __restrict__ las,
@@ -703,8 +704,8 @@ collide_kernel(const LevelArgs* __restrict__ las,
     // (the values they would give: not solid, sponge 0, no wall model); the
     // branch comes after the pull and stencil loads so that it does not hold
     // them back.  Measured on the cube case: FP32 finest collide -2 %; the
-    // 16-bit forms, limited by loads in flight rather than bytes, lose 1.4 %
-    // to the extra branch and keep the unconditional loads (bfl = 7).
+    // 16-bit forms, limited by loads in flight rather than bytes, lose 1.4 to
+    // 2.3 % to the extra branch and keep the unconditional loads (bfl = 7).
     uint32_t bfl = 7u;
     if constexpr (ST == 0) {
         bfl = a.bflags[b];
