@@ -559,40 +559,52 @@ __device__ __forceinline__ const GF* gaddr(const void* base, OffT off) {
 // -z face plane, wave 1 the +z plane, wave 2 the two x faces and wave 3 the
 // two y faces, each value through the stencil table (the neighbour block's
 // cell, or the adjacent own cell where the neighbour is outside the level).
-// 6 loads per lane instead of the 18 of a per-lane stencil.
+// 6 loads per lane instead of the 18 of a per-lane stencil.  vel_tile_load
+// issues the loads; vel_tile_store writes the tile.
 constexpr int VT_SIZE = 6 * 100;
 __device__ __forceinline__ int vt_index(int d, int zz, int yy, int xx) {
     return d * VT_SIZE + (zz + 1) * 100 + (yy + 1) * 10 + (xx + 1);
 }
+struct VelTile {
+    float o[3], h[3];
+    int hz, hy, hx;
+};
 template <class OffT>
-__device__ __forceinline__ void vel_tile(const float* vel, const OffT* vtab, int half, int c, float* vt) {
+__device__ __forceinline__ VelTile vel_tile_load(const float* vel, const OffT* vtab, int half, int c) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int x = c & 7, y = (c >> 3) & 7;
+    VelTile t;
     const GF* p = gaddr(vel, vtab[wv * 8 + 6] + (OffT)(4 * c));
-    const float o0 = p[0], o1 = p[NC], o2 = p[2 * NC];
-    int n, ce, hz, hy, hx;  // face, own edge cell, halo position
+    t.o[0] = p[0], t.o[1] = p[NC], t.o[2] = p[2 * NC];
+    int n, ce;  // face and own edge cell of this lane's halo value
     if (wv == 0) {
-        n = 5, ce = (half << 8) + lane, hz = -1, hy = lane >> 3, hx = lane & 7;
+        n = 5, ce = (half << 8) + lane, t.hz = -1, t.hy = lane >> 3, t.hx = lane & 7;
     } else if (wv == 1) {
-        n = 4, ce = (half << 8) + 192 + lane, hz = 4, hy = lane >> 3, hx = lane & 7;
+        n = 4, ce = (half << 8) + 192 + lane, t.hz = 4, t.hy = lane >> 3, t.hx = lane & 7;
     } else {
-        const int side = lane >> 5, t = (lane >> 2) & 7, zl = lane & 3;
+        const int side = lane >> 5, s = (lane >> 2) & 7, zl = lane & 3;
         if (wv == 2)
-            n = side ? 0 : 1, ce = (half << 8) + zl * 64 + 8 * t + (side ? 7 : 0), hz = zl, hy = t, hx = side ? 8 : -1;
+            n = side ? 0 : 1, ce = (half << 8) + zl * 64 + 8 * s + (side ? 7 : 0), t.hz = zl, t.hy = s,
+            t.hx = side ? 8 : -1;
         else
-            n = side ? 2 : 3, ce = (half << 8) + zl * 64 + 8 * (side ? 7 : 0) + t, hz = zl, hy = side ? 8 : -1, hx = t;
+            n = side ? 2 : 3, ce = (half << 8) + zl * 64 + 8 * (side ? 7 : 0) + s, t.hz = zl,
+            t.hy = side ? 8 : -1, t.hx = s;
     }
     const int dx = n == 0 ? 1 : (n == 1 ? -1 : 0);
     const int dy = n == 2 ? 1 : (n == 3 ? -1 : 0);
     const int dz = n == 4 ? 1 : (n == 5 ? -1 : 0);
     const GF* q = gaddr(vel, vtab[((ce >> 6) & 3) * 8 + n] + (OffT)(4 * (ce + dx + 8 * dy + 64 * dz)));
-    const float h0 = q[0], h1 = q[NC], h2 = q[2 * NC];
-    vt[vt_index(0, wv, y, x)] = o0;
-    vt[vt_index(1, wv, y, x)] = o1;
-    vt[vt_index(2, wv, y, x)] = o2;
-    vt[vt_index(0, hz, hy, hx)] = h0;
-    vt[vt_index(1, hz, hy, hx)] = h1;
-    vt[vt_index(2, hz, hy, hx)] = h2;
+    t.h[0] = q[0], t.h[1] = q[NC], t.h[2] = q[2 * NC];
+    return t;
+}
+// Second half: the values into the tile, once the caller has issued its
+// population loads, so that the wait and the barrier overlap them.
+__device__ __forceinline__ void vel_tile_store(const VelTile& t, int c, float* vt) {
+    const int wv = threadIdx.x >> 6, x = c & 7, y = (c >> 3) & 7;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        vt[vt_index(d, wv, y, x)] = t.o[d];
+        vt[vt_index(d, t.hz, t.hy, t.hx)] = t.h[d];
+    }
 }
 
 // stream_collide_cumulant! (src/collision/kernel_cumulant.jl:29-627), with the
@@ -622,13 +634,14 @@ collide_kernel(const LevelArgs* __restrict__ las,
     // interior blocks: outputs leave through stream_tile (the boundary
     // kernel's cascade buffer already takes the LDS budget)
     constexpr bool STAGE = !BND;
-    // FP32 interior blocks: the work-group's previous velocity with its face
-    // halos goes through LDS for the WALE stencil (vel_tile), and the output
-    // tile is halved so that both fit 5 work-groups per CU.  Measured on the
-    // 52.5M-cell cube case, finest collide: FP32 -1.9 % with the velocity
-    // tile; FP16 +1.5 % (the work-group barrier), so 16-bit storage loads the
-    // stencil per lane.
-    constexpr bool VTILE = STAGE && ST == 0;
+    // Interior blocks: the work-group's previous velocity with its face halos
+    // goes through LDS for the WALE stencil (vel_tile_load / _store), and in
+    // FP32 the output tile is halved so that both fit 5 work-groups per CU.
+    // The tile's loads are issued before the pull and stored after it, so
+    // its barrier overlaps the pull's loads.  Measured on the 52.5M-cell cube
+    // case, finest collide: FP32 -1.9 %, FP16 -3.1 % (with the tile loads
+    // issued after the pull, FP16 lost 1.5 % to the barrier).
+    constexpr bool VTILE = STAGE;
     constexpr int ROWS = ST == 0 ? 14 : 27;
     __shared__ __attribute__((aligned(16))) Stored<ST> sst[STAGE ? 4 * ROWS * 64 : 8];
     __shared__ float vt[VTILE ? 3 * VT_SIZE : 1];
@@ -660,6 +673,8 @@ collide_kernel(const LevelArgs* __restrict__ las,
     const int oxp = x == 0 ? -1 : 0, oxm = x == 7 ? 1 : 0;
     const int oyp = y == 0 ? -1 : 0, oym = y == 7 ? 1 : 0;
     const OffT cb = (OffT)c * (OffT)ESZ;
+    VelTile vtl;
+    if constexpr (VTILE) vtl = vel_tile_load<OffT>(a.vel_in, vtab, half, c);
     float A[27];
 #pragma unroll
     for (int k = 0; k < 27; k++) {
@@ -673,7 +688,7 @@ collide_kernel(const LevelArgs* __restrict__ las,
     // WALE stencil of the previous velocity, loaded together with the populations.
     float VN[6][3];
     if constexpr (VTILE) {
-        vel_tile<OffT>(a.vel_in, vtab, half, c, vt);
+        vel_tile_store(vtl, c, vt);
         __syncthreads();
 #pragma unroll
         for (int n = 0; n < 6; n++) {
