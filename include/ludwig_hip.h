@@ -55,7 +55,11 @@ typedef struct {
     float q_min_threshold;                   /* Q_MIN_THRESHOLD                          */
     int32_t storage;                         /* LB_STORAGE_*                             */
     int32_t strict_fp;                       /* 1: no FMA contraction (bitwise-comparable) */
-    int32_t reserved[7];
+    int32_t wide_offsets;                    /* 1: 64-bit population offsets on every level.
+                                              * A level switches to them by itself once its
+                                              * population buffer reaches 4 GiB; forcing them
+                                              * runs those kernel instances on any case.     */
+    int32_t reserved[6];
 } lb_config;
 
 /* One BlockLevel (src/geometry/blocks.jl:16-74) as the reference holds it. */
@@ -192,11 +196,20 @@ int lb_surface_maps(lb_ctx* ctx, const lb_force_params* fp, int32_t side, float*
 int lb_download_level(lb_ctx* ctx, int level_id, int field, float* dst);
 int lb_upload_level(lb_ctx* ctx, int level_id, int field, const float* src);
 
-/* FP64 diagnostics over fluid cells of a level (diagnostics.jl:43-93, plus
- * the global mass/energy check): out[0] sum rho, [1] sum 0.5 rho u^2,
- * [2] rho min, [3] rho max, [4] number of non-finite populations,
- * [5] fluid cell count, [6] sum of all populations (mass), [7] reserved.  */
+/* FP64 diagnostics over fluid cells of a level (compute_flow_stats,
+ * diagnostics.jl:43-93, plus the global mass/energy check): out[0] sum rho,
+ * [1] sum 0.5 rho u^2, [2] rho min, [3] rho max, [4] number of non-finite
+ * populations, [5] fluid cell count, [6] sum of all populations (mass),
+ * [7] v_max = max |u| over fluid cells.                                     */
 int lb_stats(lb_ctx* ctx, int level_id, double* out);
+
+/* Divergence check (check_stability, diagnostics.jl:95-121, which only
+ * prints warnings; here a diverged case is an error): counts non-finite
+ * density and velocity values of every level after the last completed
+ * coarse step.  Returns LB_ENONFINITE (the count in lb_last_error) when any
+ * is found, else LB_OK; *n_nonfinite (may be NULL) receives the count.
+ * Reads 16 bytes per cell; meant for the diagnostic cadence.               */
+int lb_check_finite(lb_ctx* ctx, int64_t* n_nonfinite);
 
 /* Device bytes the context holds (serving admission).                    */
 int64_t lb_device_bytes(const lb_ctx* ctx);
@@ -258,6 +271,28 @@ int lb_engine_synchronize(lb_engine* e);
 
 /* Coarse steps a case has completed once the enqueued work has run. */
 int lb_engine_case_progress(const lb_engine* e, const lb_ctx* ctx, int64_t* t_done);
+
+/* Force sampling of an attached case inside the engine step
+ * (accumulate_forces!, surface.jl:36-56, at the samples main.jl:236-243
+ * takes): at the end of every coarse step t with t % diag_freq == 0 and
+ * t > ramp_steps, the finest level's forces -- as lb_forces computes them
+ * with fp -- are integrated and added to the case's FP64 accumulator.  The
+ * case needs a mesh (lb_mesh_upload).  A case's samples do not depend on the
+ * other attached cases.                                                     */
+int lb_engine_sample_forces(lb_engine* e, lb_ctx* ctx, const lb_force_params* fp, int64_t diag_freq,
+                            int64_t ramp_steps);
+/* The accumulator (waits for the engine): sums[0..8] = sums over the samples
+ * of Fp xyz, Fv xyz, M xyz (the LB_F_FXP..LB_F_MZ slots of lb_forces, with
+ * its symmetric doubling), *n_samples their number.  Read it before
+ * lb_engine_remove.                                                        */
+int lb_engine_case_forces(lb_engine* e, const lb_ctx* ctx, double* sums, int64_t* n_samples);
+
+/* The divergence check of lb_check_finite over every attached case, after
+ * the enqueued work (waits for it).  The first `cap` diverged cases are
+ * written to diverged[] and their number to *n_diverged; returns
+ * LB_ENONFINITE if there is any.  The other cases are not affected: a
+ * diverged case can be removed (lb_engine_remove) while they go on.       */
+int lb_engine_check_finite(lb_engine* e, lb_ctx** diverged, int32_t cap, int32_t* n_diverged);
 
 /* Launch options (picked by the autotuner; results do not depend on them):
  * graph (1: replay a captured hipGraph per step), the work-group schedule

@@ -14,6 +14,7 @@ import numpy as np
 
 _BUILD = Path(__file__).resolve().parent / "_build"
 
+LB_OK, LB_EINVAL, LB_EHIP, LB_ENOMEM, LB_ESTATE, LB_ENONFINITE = 0, -1, -2, -3, -4, -5
 LB_STORAGE_FP32, LB_STORAGE_FP16, LB_STORAGE_BF16 = 0, 1, 2
 LB_FIELD_F, LB_FIELD_RHO, LB_FIELD_VEL, LB_FIELD_G, LB_FIELD_T = 0, 1, 2, 3, 4
 LB_F_N = 18
@@ -33,7 +34,7 @@ class LbConfig(C.Structure):
     _fields_ = [("n_levels", C.c_int32), ("domain_nx", C.c_int32), ("domain_ny", C.c_int32),
                 ("domain_nz", C.c_int32), ("temporal_interpolation", C.c_int32), ("ddf_enabled", C.c_int32),
                 ("q_min_threshold", C.c_float), ("storage", C.c_int32), ("strict_fp", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("wide_offsets", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class LbLevelDesc(C.Structure):
@@ -80,6 +81,7 @@ HIP_EXPORTS = {
     "lb_download_level": (C.c_int, [C.c_void_p, C.c_int, C.c_int, f32p]),
     "lb_upload_level": (C.c_int, [C.c_void_p, C.c_int, C.c_int, f32p]),
     "lb_stats": (C.c_int, [C.c_void_p, C.c_int, f64p]),
+    "lb_check_finite": (C.c_int, [C.c_void_p, i64p]),
     "lb_device_bytes": (C.c_int64, [C.c_void_p]),
     "lb_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "lb_profile_read": (C.c_int, [C.c_void_p, C.c_int, f64p, i64p]),
@@ -94,6 +96,9 @@ HIP_EXPORTS = {
     "lb_engine_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "lb_engine_synchronize": (C.c_int, [C.c_void_p]),
     "lb_engine_case_progress": (C.c_int, [C.c_void_p, C.c_void_p, i64p]),
+    "lb_engine_sample_forces": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LbForceParams), C.c_int64, C.c_int64]),
+    "lb_engine_case_forces": (C.c_int, [C.c_void_p, C.c_void_p, f64p, i64p]),
+    "lb_engine_check_finite": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_int32)]),
     "lb_engine_set_options": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "lb_footprint": (C.c_int, [C.POINTER(LbConfig), C.POINTER(LbLevelDesc), C.c_int32, C.c_int64, C.c_int32,
                                i64p]),

@@ -54,11 +54,14 @@ struct Kern {
     hipError_t (*forces)(const lbk::ForceArgs&, int, double*, hipStream_t);
     hipError_t (*stats)(const void*, const float*, const float*, const uint8_t*, int32_t, int, double*, int,
                         hipStream_t);
+    hipError_t (*finite_count)(const lbk::FiniteEntry*, int32_t, int32_t, int32_t*, hipStream_t);
+    hipError_t (*engine_forces)(const lbk::EngineForce*, const lbk::CaseState*, const lbk::Work*, int32_t,
+                                const int32_t*, int32_t, int64_t, hipStream_t);
 };
 
 #define LBK_TABLE(NS)                                                                                     \
     {NS::collide, NS::thermal, NS::ghost_fill, NS::bouzidi, NS::set_state, NS::advance, NS::init_level,  \
-     NS::init_thermal, NS::forces, NS::stats}
+     NS::init_thermal, NS::forces, NS::stats, NS::finite_count, NS::engine_forces}
 const Kern kStrict = LBK_TABLE(lbk_strict);
 const Kern kFast = LBK_TABLE(lbk_fast);
 
@@ -132,6 +135,9 @@ struct lb_ctx {
     double* fpart = nullptr;
     double* spart = nullptr;
     int n_stat_groups = 0;
+    // divergence check: one entry per level and a counter (allocated at creation)
+    lbk::FiniteEntry* fin_ents = nullptr;
+    int32_t* fin_count = nullptr;
     bool profile = false;
     // device-resident kernel arguments: the case record and one LevelArgs per level
     lbk::CaseState* st_dev = nullptr;
@@ -150,6 +156,11 @@ struct EngineCase {
     int32_t batch = 8;
     float* u_tab = nullptr;  // device, owned by the case context
     lb_step_params p{};
+    // force sampling inside the engine step (lb_engine_sample_forces)
+    double* acc = nullptr;   // device [10], owned by the case context
+    bool sample = false;
+    int64_t diag = 0, ramp = 0;
+    lb_force_params fp{};
 };
 
 // Work lists of one depth of the engine's recursion (concatenated over cases).
@@ -159,6 +170,11 @@ struct DepthWork {
               *tlk = nullptr;
     int32_t ncol = 0, ncolb = 0, ngh = 0, nlk = 0, ntcol = 0, ntcolb = 0, ntgh = 0, ntlk = 0;
     int wide = 0, wide_g = 0;
+    // force sampling of the cases whose first level runs at this depth:
+    // triangle chunks (li = record, item = chunk) and the record indices
+    lbk::Work* fw = nullptr;
+    int32_t* frec = nullptr;
+    int32_t nfw = 0, nfrec = 0;
 };
 
 struct lb_engine {
@@ -174,6 +190,7 @@ struct lb_engine {
     std::vector<void*> allocs;
     lbk::LevelArgs* las = nullptr;
     lbk::CaseState* states = nullptr;  // one record per attached case, contiguous
+    lbk::EngineForce* frecs = nullptr; // force-sampling records
     std::vector<DepthWork> depth;
     int32_t n_active = 0;
     // options
@@ -182,6 +199,10 @@ struct lb_engine {
     int waves = 5;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // divergence check buffers (grown on demand, kept across rebuilds)
+    lbk::FiniteEntry* fin_ents = nullptr;
+    int32_t* fin_counts = nullptr;
+    size_t fin_cap_ents = 0, fin_cap_cases = 0;
 };
 
 namespace {
@@ -605,6 +626,7 @@ void engine_free_lists(lb_engine* e) {
     e->allocs.clear();
     e->las = nullptr;
     e->states = nullptr;
+    e->frecs = nullptr;
     e->depth.clear();
 }
 
@@ -617,6 +639,8 @@ int eupload(lb_engine* e, T** dst, const std::vector<T>& v) {
     EHIPCHK(e, hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     return LB_OK;
 }
+
+lbk::ForceArgs force_args(lb_ctx* c, const lb_force_params* fp, float side);
 
 // The device record of an attached case at its current position.
 lbk::CaseState engine_record(const lb_engine* e, const EngineCase& ec) {
@@ -649,7 +673,42 @@ int engine_rebuild(lb_engine* e) {
     }
     RC(eupload(e, &e->las, las));
     RC(eupload(e, &e->states, sts));
+    // force sampling: one record per sampling case, its triangle chunks
+    // listed at the depth of the case's first level
+    std::vector<lbk::EngineForce> frecs;
+    std::vector<std::vector<lbk::Work>> fw((size_t)e->lmax);
+    std::vector<std::vector<int32_t>> frec((size_t)e->lmax);
+    for (size_t i = 0; i < e->cases.size(); i++) {
+        EngineCase& ec = e->cases[i];
+        if (!ec.sample) continue;
+        lb_ctx* c = ec.c;
+        const Level& fin = c->levels.back();
+        lbk::EngineForce r;
+        std::memset((void*)&r, 0, sizeof r);
+        r.a = force_args(c, &ec.fp, 1.0f);
+        r.rho2[0] = fin.rho[0];
+        r.rho2[1] = fin.rho[1];
+        r.diag = ec.diag;
+        r.ramp = ec.ramp;
+        r.ci = (int32_t)i;
+        r.leaf_shift = (int32_t)c->levels.size() - 1;
+        r.n_groups = (int32_t)((c->n_tri + 255) / 256);
+        r.acc = ec.acc;
+        const int d = e->lmax - (int)c->levels.size() + 1;
+        const int32_t ri = (int32_t)frecs.size();
+        for (int32_t g = 0; g < r.n_groups; g++) fw[(size_t)d - 1].push_back(mkwork(ri, g, (int32_t)i, nullptr));
+        frec[(size_t)d - 1].push_back(ri);
+        frecs.push_back(r);
+    }
+    RC(eupload(e, &e->frecs, frecs));
     e->depth.assign((size_t)e->lmax, DepthWork());
+    for (int d = 1; d <= e->lmax; d++) {
+        DepthWork& W = e->depth[(size_t)d - 1];
+        RC(eupload(e, &W.fw, fw[(size_t)d - 1]));
+        RC(eupload(e, &W.frec, frec[(size_t)d - 1]));
+        W.nfw = (int32_t)fw[(size_t)d - 1].size();
+        W.nfrec = (int32_t)frec[(size_t)d - 1].size();
+    }
     for (int d = 1; d <= e->lmax; d++) {
         std::vector<lbk::Work> col, colb, gh, lk, tcol, tcolb, tgh, tlk;
         DepthWork& W = e->depth[(size_t)d - 1];
@@ -732,6 +791,8 @@ hipError_t engine_op(lb_engine* e, int d, int64_t j) {
         ETRY(engine_op(e, d + 1, 2 * j));
         ETRY(engine_op(e, d + 1, 2 * j + 1));
     }
+    // coarse step t0 + j of the cases whose first level runs at depth d is complete
+    if (W.nfw) ETRY(k->engine_forces(e->frecs, st, W.fw, W.nfw, W.frec, W.nfrec, j, e->stream));
 #undef ETRY
     return r;
 }
@@ -740,6 +801,19 @@ hipError_t engine_issue_step(lb_engine* e) {
     hipError_t r = engine_op(e, 1, 0);
     if (r != hipSuccess) return r;
     return e->k->advance(e->states, (int32_t)e->cases.size(), e->stream);
+}
+
+// Divergence check entry of one level: its density and velocity in buffer
+// `cur` (the output of the last completed coarse step).
+constexpr int32_t kFiniteGroups = 64;  // work-groups per (case, level) entry
+lbk::FiniteEntry finite_entry(const Level& L, int cur, int32_t slot) {
+    lbk::FiniteEntry e;
+    e.rho = L.rho[cur];
+    e.vel = L.vel[cur];
+    e.n = (int64_t)L.nb * NC;
+    e.slot = slot;
+    e.pad = 0;
+    return e;
 }
 
 int check_ctx(lb_ctx* c) {
@@ -790,7 +864,8 @@ int lb_create(lb_ctx** out, int device, const lb_config* cfg) {
     c->levels.resize((size_t)cfg->n_levels);
     c->las_host.resize((size_t)cfg->n_levels);
     std::memset((void*)c->las_host.data(), 0, c->las_host.size() * sizeof(lbk::LevelArgs));
-    if (dalloc(c, &c->st_dev, 1) != LB_OK || dalloc(c, &c->las_dev, (size_t)cfg->n_levels) != LB_OK) {
+    if (dalloc(c, &c->st_dev, 1) != LB_OK || dalloc(c, &c->las_dev, (size_t)cfg->n_levels) != LB_OK ||
+        dalloc(c, &c->fin_ents, (size_t)cfg->n_levels) != LB_OK || dalloc(c, &c->fin_count, 1) != LB_OK) {
         lb_destroy(c);
         return LB_ENOMEM;
     }
@@ -932,7 +1007,7 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
         }
     }
     const size_t n4 = (size_t)nb * NC, n5 = ((size_t)nb + ngh) * NC * Q;
-    L.wide = (n5 + 512) * pop_bytes(c->cfg.storage) >= ((size_t)1 << 32);
+    L.wide = c->cfg.wide_offsets != 0 || (n5 + 512) * pop_bytes(c->cfg.storage) >= ((size_t)1 << 32);
     L.wide_g = L.wide;  // thermal populations use the same storage
     for (int i = 0; i < 2; i++) {
         // 512 slots of padding: pulls from a missing neighbour block read a
@@ -1315,19 +1390,19 @@ int lb_stats(lb_ctx* c, int level_id, double* out) {
     const Level& L = c->levels[(size_t)level_id - 1];
     const int groups = std::max(1, std::min<int>(L.nb, 1024));
     if (c->n_stat_groups < groups) {
-        RC(dalloc(c, &c->spart, (size_t)1024 * 7));
+        RC(dalloc(c, &c->spart, (size_t)1024 * 8));
         c->n_stat_groups = 1024;
     }
     for (int i = 0; i < 8; i++) out[i] = 0;
     if (L.nb == 0) return LB_OK;
     HIPCHK(c, c->k->stats(L.f[L.cur], L.rho[L.cur], L.vel[L.cur], L.obst, L.nb, c->cfg.storage, c->spart, groups,
                           c->stream));
-    std::vector<double> part((size_t)groups * 7);
+    std::vector<double> part((size_t)groups * 8);
     HIPCHK(c, hipMemcpyAsync(part.data(), c->spart, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    double mn = 1e300, mx = -1e300;
+    double mn = 1e300, mx = -1e300, vmax = 0;
     for (int g = 0; g < groups; g++) {
-        const double* q = &part[(size_t)g * 7];
+        const double* q = &part[(size_t)g * 8];
         out[0] += q[0];
         out[1] += q[1];
         mn = std::min(mn, q[2]);
@@ -1335,9 +1410,32 @@ int lb_stats(lb_ctx* c, int level_id, double* out) {
         out[4] += q[4];
         out[5] += q[5];
         out[6] += q[6];
+        vmax = std::max(vmax, q[7]);
     }
     out[2] = mn;
     out[3] = mx;
+    out[7] = vmax;
+    return LB_OK;
+}
+
+int lb_check_finite(lb_ctx* c, int64_t* n_nonfinite) {
+    lbsync::SetupGuard guard;
+    RC(check_ctx(c));
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<lbk::FiniteEntry> ents;
+    for (const Level& L : c->levels)
+        if (L.nb > 0) ents.push_back(finite_entry(L, L.cur, 0));
+    int32_t count = 0;
+    if (!ents.empty()) {
+        HIPCHK(c, hipMemcpyAsync(c->fin_ents, ents.data(), ents.size() * sizeof(lbk::FiniteEntry),
+                                 hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->fin_count, 0, sizeof(int32_t), c->stream));
+        HIPCHK(c, c->k->finite_count(c->fin_ents, (int32_t)ents.size(), kFiniteGroups, c->fin_count, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&count, c->fin_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (n_nonfinite) *n_nonfinite = count;
+    if (count > 0) return fail(c, LB_ENONFINITE, "%d non-finite density/velocity values", count);
     return LB_OK;
 }
 
@@ -1396,6 +1494,8 @@ void lb_engine_destroy(lb_engine* e) {
     (void)hipStreamSynchronize(e->stream);
     for (EngineCase& ec : e->cases) ec.c->engine = nullptr;
     engine_free_lists(e);
+    if (e->fin_ents) (void)hipFree(e->fin_ents);
+    if (e->fin_counts) (void)hipFree(e->fin_counts);
     (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -1422,6 +1522,8 @@ int lb_engine_add(lb_engine* e, lb_ctx* c, int64_t n_coarse_steps, const float* 
     ec.p = *p;
     if (dalloc(c, &ec.u_tab, (size_t)n_batches) != LB_OK) return efail(e, LB_ENOMEM, "%s", c->err.c_str());
     if (h2d(c, ec.u_tab, u_batches, (size_t)n_batches) != LB_OK) return efail(e, LB_EHIP, "%s", c->err.c_str());
+    if (dalloc(c, &ec.acc, 10) != LB_OK) return efail(e, LB_ENOMEM, "%s", c->err.c_str());
+    EHIPCHK(e, hipMemset(ec.acc, 0, 10 * sizeof(double)));
     c->engine = e;
     e->cases.push_back(ec);
     e->dirty = true;
@@ -1495,6 +1597,96 @@ int lb_engine_case_progress(const lb_engine* e, const lb_ctx* c, int64_t* t_done
     return LB_EINVAL;
 }
 
+int lb_engine_sample_forces(lb_engine* e, lb_ctx* c, const lb_force_params* fp, int64_t diag_freq,
+                            int64_t ramp_steps) {
+    lbsync::SetupGuard guard;
+    if (!e || !c || !fp || diag_freq < 1 || ramp_steps < 0) return LB_EINVAL;
+    for (EngineCase& ec : e->cases) {
+        if (ec.c != c) continue;
+        if (c->n_tri <= 0) return efail(e, LB_ESTATE, "case has no mesh");
+        EHIPCHK(e, hipSetDevice(e->device));
+        EHIPCHK(e, hipStreamSynchronize(e->stream));
+        ec.sample = true;
+        ec.diag = diag_freq;
+        ec.ramp = ramp_steps;
+        ec.fp = *fp;
+        e->dirty = true;
+        return LB_OK;
+    }
+    return efail(e, LB_EINVAL, "case is not attached to this engine");
+}
+
+int lb_engine_case_forces(lb_engine* e, const lb_ctx* c, double* sums, int64_t* n_samples) {
+    lbsync::SetupGuard guard;
+    if (!e || !c || !sums || !n_samples) return LB_EINVAL;
+    for (const EngineCase& ec : e->cases) {
+        if (ec.c != c) continue;
+        EHIPCHK(e, hipSetDevice(e->device));
+        EHIPCHK(e, hipStreamSynchronize(e->stream));
+        double acc[10];
+        EHIPCHK(e, hipMemcpy(acc, ec.acc, sizeof acc, hipMemcpyDeviceToHost));
+        for (int i = 0; i < 9; i++) sums[i] = acc[i];
+        // integrate_surface_forces! (surface.jl:873-910) doubles the half model
+        if (ec.fp.symmetric) {
+            sums[0] *= 2.0; sums[2] *= 2.0; sums[3] *= 2.0; sums[5] *= 2.0; sums[7] *= 2.0;
+            sums[1] = 0.0; sums[4] = 0.0; sums[6] = 0.0; sums[8] = 0.0;
+        }
+        *n_samples = (int64_t)acc[9];
+        return LB_OK;
+    }
+    return efail(e, LB_EINVAL, "case is not attached to this engine");
+}
+
+int lb_engine_check_finite(lb_engine* e, lb_ctx** diverged, int32_t cap, int32_t* n_diverged) {
+    lbsync::SetupGuard guard;
+    if (!e || !n_diverged || cap < 0 || (cap > 0 && !diverged)) return LB_EINVAL;
+    *n_diverged = 0;
+    EHIPCHK(e, hipSetDevice(e->device));
+    EHIPCHK(e, hipStreamSynchronize(e->stream));
+    const size_t nc = e->cases.size();
+    if (nc == 0) return LB_OK;
+    std::vector<lbk::FiniteEntry> ents;
+    for (size_t i = 0; i < nc; i++) {
+        const EngineCase& ec = e->cases[i];
+        const int64_t done = std::min(ec.t0 - 1, ec.n_total);
+        for (size_t li = 0; li < ec.c->levels.size(); li++) {
+            const Level& L = ec.c->levels[li];
+            if (L.nb == 0) continue;
+            // output buffer of the level's last substep in coarse step `done`
+            const int64_t last = (done << li) + ((int64_t)1 << li) - 1;
+            ents.push_back(finite_entry(L, done >= 1 ? 1 - (int)(last & 1) : L.cur, (int32_t)i));
+        }
+    }
+    if (ents.size() > e->fin_cap_ents || nc > e->fin_cap_cases) {
+        if (e->fin_ents) (void)hipFree(e->fin_ents);
+        if (e->fin_counts) (void)hipFree(e->fin_counts);
+        e->fin_ents = nullptr;
+        e->fin_counts = nullptr;
+        e->fin_cap_ents = e->fin_cap_cases = 0;
+        const size_t ce = std::max<size_t>(ents.size(), 64), cc = std::max<size_t>(nc, 16);
+        EHIPCHK(e, hipMalloc((void**)&e->fin_ents, ce * sizeof(lbk::FiniteEntry)));
+        EHIPCHK(e, hipMalloc((void**)&e->fin_counts, cc * sizeof(int32_t)));
+        e->fin_cap_ents = ce;
+        e->fin_cap_cases = cc;
+    }
+    std::vector<int32_t> counts(nc, 0);
+    EHIPCHK(e, hipMemcpyAsync(e->fin_ents, ents.data(), ents.size() * sizeof(lbk::FiniteEntry), hipMemcpyHostToDevice,
+                              e->stream));
+    EHIPCHK(e, hipMemsetAsync(e->fin_counts, 0, nc * sizeof(int32_t), e->stream));
+    EHIPCHK(e, e->k->finite_count(e->fin_ents, (int32_t)ents.size(), kFiniteGroups, e->fin_counts, e->stream));
+    EHIPCHK(e, hipMemcpyAsync(counts.data(), e->fin_counts, nc * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+    EHIPCHK(e, hipStreamSynchronize(e->stream));
+    int32_t n = 0;
+    for (size_t i = 0; i < nc; i++)
+        if (counts[i] > 0) {
+            if (n < cap) diverged[n] = e->cases[i].c;
+            n++;
+        }
+    *n_diverged = n;
+    if (n > 0) return efail(e, LB_ENONFINITE, "%d case(s) hold non-finite density/velocity values", n);
+    return LB_OK;
+}
+
 int lb_engine_set_options(lb_engine* e, int32_t use_graph, int32_t schedule, int32_t waves) {
     if (!e || (schedule != LB_SCHED_XCD && schedule != LB_SCHED_LINEAR) || waves < 5 || waves > 6) return LB_EINVAL;
     EHIPCHK(e, hipSetDevice(e->device));
@@ -1513,7 +1705,7 @@ int lb_engine_set_options(lb_engine* e, int32_t use_graph, int32_t schedule, int
 // Device bytes a case will hold: the allocations of lb_create,
 // lb_level_upload (every level) and lb_mesh_upload, counted by running the
 // same host code without allocating (no GPU needed), plus the engine's ramp
-// table of n_ramp_batches values.  lb_stats adds 57,344 bytes on first use.
+// table of n_ramp_batches values.  lb_stats adds 65,536 bytes on first use.
 int lb_footprint(const lb_config* cfg, const lb_level_desc* levels, int32_t n_levels, int64_t n_tri,
                  int32_t n_ramp_batches, int64_t* bytes) {
     if (!cfg || !levels || !bytes || n_levels != cfg->n_levels || n_levels < 1 || n_levels > 16) return LB_EINVAL;
@@ -1524,11 +1716,15 @@ int lb_footprint(const lb_config* cfg, const lb_level_desc* levels, int32_t n_le
     c.las_host.resize((size_t)n_levels);
     RC(dalloc(&c, &c.st_dev, 1));
     RC(dalloc(&c, &c.las_dev, (size_t)n_levels));
+    RC(dalloc(&c, &c.fin_ents, (size_t)n_levels));
+    RC(dalloc(&c, &c.fin_count, 1));
     for (int i = 0; i < n_levels; i++) RC(lb_level_upload(&c, &levels[i]));
     if (n_tri > 0) RC(lb_mesh_upload(&c, n_tri, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
     if (n_ramp_batches > 0) {
         float* u = nullptr;
+        double* acc = nullptr;
         RC(dalloc(&c, &u, (size_t)n_ramp_batches));
+        RC(dalloc(&c, &acc, 10));
     }
     *bytes = c.bytes;
     return LB_OK;

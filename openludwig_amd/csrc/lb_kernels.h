@@ -139,6 +139,33 @@ struct ForceArgs {
     float side;        // +1: probe along the outward normal; -1: the minus side
 };
 
+// Force sampling of one case attached to the batched engine: the finest
+// level's force arguments (a.rho is replaced by the density of the sampled
+// step's parity), the sampling cadence and the FP64 accumulator
+// acc[0..8] = sums of the nine force/moment components, acc[9] = samples.
+struct EngineForce {
+    ForceArgs a;
+    const float* rho2[2];  // finest level's density pair
+    int64_t diag;          // sample every diag coarse steps ...
+    int64_t ramp;          // ... after the ramp (t > ramp)
+    int32_t ci;            // CaseState index
+    int32_t leaf_shift;    // finest level_id - 1
+    int32_t n_groups;      // triangle chunks of 256 (partials rows)
+    int32_t pad;
+    double* acc;
+};
+
+// One (case, level) of the divergence check: the level's current density
+// (n values) and velocity (3n values); non-finite values are added to
+// counts[slot].
+struct FiniteEntry {
+    const float* rho;
+    const float* vel;
+    int64_t n;
+    int32_t slot;
+    int32_t pad;
+};
+
 }  // namespace lbk
 
 #define LBK_DECLARE(NS)                                                                                  \
@@ -162,6 +189,10 @@ struct ForceArgs {
     hipError_t forces(const lbk::ForceArgs& a, int n_groups, double* out9, hipStream_t s);               \
     hipError_t stats(const void* f, const float* rho, const float* vel, const uint8_t* obst, int32_t nb,  \
                      int storage, double* partials, int n_groups, hipStream_t s);                        \
+    hipError_t finite_count(const lbk::FiniteEntry* ents, int32_t n_ents, int32_t groups_per_entry,          \
+                            int32_t* counts, hipStream_t s);                                             \
+    hipError_t engine_forces(const lbk::EngineForce* recs, const lbk::CaseState* states, const lbk::Work* work,  \
+                             int32_t n_chunks, const int32_t* rec_idx, int32_t n_recs, int64_t j, hipStream_t s); \
     }
 
 LBK_DECLARE(lbk_strict)

@@ -1689,9 +1689,11 @@ __device__ bool trilinear_sample(const ForceArgs& a, float px, float py, float p
 // fallback's side test, the wall distance and the stress (each a sign flip,
 // exact), so the maps are computed here with n -> -n; the integrated sums
 // are then not used.
-__global__ void __launch_bounds__(256) forces_kernel(const ForceArgs a) {
+// One work-group: triangles chunk*256 .. chunk*256+255, partial sums into
+// a.partials[chunk].
+__device__ void forces_block(const ForceArgs& a, int64_t chunk) {
     __shared__ double red[9][256];
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = chunk * 256 + threadIdx.x;
     float d9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (i < a.n_tri) {
         float tx = a.cx[i] + a.off_x, ty = a.cy[i] + a.off_y, tz = a.cz[i] + a.off_z;
@@ -1792,8 +1794,10 @@ __global__ void __launch_bounds__(256) forces_kernel(const ForceArgs a) {
             for (int j = 0; j < 9; j++) red[j][threadIdx.x] += red[j][threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x < 9) a.partials[(size_t)blockIdx.x * 9 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 9) a.partials[(size_t)chunk * 9 + threadIdx.x] = red[threadIdx.x][0];
 }
+
+__global__ void __launch_bounds__(256) forces_kernel(const ForceArgs a) { forces_block(a, blockIdx.x); }
 
 hipError_t forces(const ForceArgs& a, int n_groups, double* out9, hipStream_t s) {
     (void)out9;
@@ -1802,15 +1806,66 @@ hipError_t forces(const ForceArgs& a, int n_groups, double* out9, hipStream_t s)
     return hipGetLastError();
 }
 
+// Force samples inside the batched engine's step (accumulate_forces!,
+// surface.jl:36-56): coarse step t = t0 + j of the cases whose first level
+// runs at this depth is complete at this point of the recursion.  When t is
+// a sampling step (t % diag == 0, ramp < t <= n_total) the finest level's
+// forces are integrated exactly as lb_forces does -- the density of the
+// level's last substep of t, level.vel -- into each case's partials, and
+// the reduction adds the nine sums, in chunk order, to the case's
+// accumulator (so a case's samples do not depend on the other cases).
+__device__ __forceinline__ bool sample_step(const EngineForce& R, const CaseState& S, int64_t j, int64_t* t_out) {
+    const int64_t t = S.t0 + j;
+    *t_out = t;
+    return S.active && t <= S.n_total && t > R.ramp && (t % R.diag) == 0;
+}
+
+__global__ void __launch_bounds__(256) engine_forces_kernel(const EngineForce* __restrict__ recs,
+                                                            const CaseState* __restrict__ states,
+                                                            const Work* __restrict__ work, int64_t j) {
+    const Work wk = work[blockIdx.x];
+    const EngineForce& R = recs[wk.li];
+    int64_t t;
+    if (!sample_step(R, states[wk.ci], j, &t)) return;
+    ForceArgs a = R.a;
+    const int64_t last = (t << R.leaf_shift) + ((int64_t)1 << R.leaf_shift) - 1;
+    a.rho = R.rho2[1 - (int)(last & 1)];
+    forces_block(a, wk.item);
+}
+
+__global__ void engine_forces_reduce(const EngineForce* __restrict__ recs, const CaseState* __restrict__ states,
+                                     const int32_t* __restrict__ rec_idx, int32_t n, int64_t j) {
+    const int32_t q = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= n) return;
+    const EngineForce& R = recs[rec_idx[q]];
+    int64_t t;
+    if (!sample_step(R, states[R.ci], j, &t)) return;
+    double s9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int32_t g = 0; g < R.n_groups; g++)
+        for (int c = 0; c < 9; c++) s9[c] += R.a.partials[(size_t)g * 9 + c];
+    for (int c = 0; c < 9; c++) R.acc[c] += s9[c];
+    R.acc[9] += 1.0;
+}
+
+hipError_t engine_forces(const EngineForce* recs, const CaseState* states, const Work* work, int32_t n_chunks,
+                         const int32_t* rec_idx, int32_t n_recs, int64_t j, hipStream_t s) {
+    if (n_chunks <= 0 || n_recs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(engine_forces_kernel, dim3((unsigned)n_chunks), dim3(256), 0, s, recs, states, work, j);
+    hipLaunchKernelGGL(engine_forces_reduce, dim3((unsigned)((n_recs + 63) / 64)), dim3(64), 0, s, recs, states,
+                       rec_idx, n_recs, j);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
-// FP64 diagnostics per block: sum rho, KE, min/max rho over fluid cells,
-// non-finite populations, fluid count, sum of populations.
+// FP64 diagnostics per block (compute_flow_stats, diagnostics.jl:43-93, plus
+// the global mass check): sum rho, KE, min/max rho and max |u| over fluid
+// cells, non-finite populations, fluid count, sum of populations.
 template <int ST>
 __global__ void __launch_bounds__(256) stats_kernel(const void* __restrict__ f, const float* __restrict__ rho,
                                                     const float* __restrict__ vel, const uint8_t* __restrict__ obst,
                                                     int32_t nb, double* partials) {
-    __shared__ double red[7][256];
-    double s_rho = 0, s_ke = 0, mn = 1e300, mx = -1e300, nonfin = 0, nfl = 0, s_f = 0;
+    __shared__ double red[8][256];
+    double s_rho = 0, s_ke = 0, mn = 1e300, mx = -1e300, nonfin = 0, nfl = 0, s_f = 0, vmax = 0;
     for (int64_t blk = blockIdx.x; blk < nb; blk += gridDim.x) {
         for (int c = threadIdx.x; c < NC; c += 256) {
             size_t i4 = (size_t)blk * NC + c;
@@ -1825,16 +1880,19 @@ __global__ void __launch_bounds__(256) stats_kernel(const void* __restrict__ f, 
                 double r = rho[i4];
                 size_t iv = (size_t)blk * 3 * NC + c;
                 double u = vel[iv], v = vel[iv + NC], w = vel[iv + 2 * NC];
+                const double u2 = u * u + v * v + w * w;
                 s_rho += r;
-                s_ke += 0.5 * r * (u * u + v * v + w * w);
+                s_ke += 0.5 * r * u2;
                 mn = r < mn ? r : mn;
                 mx = r > mx ? r : mx;
+                const double um = sqrt(u2);
+                vmax = um > vmax ? um : vmax;
                 nfl += 1;
             }
         }
     }
     red[0][threadIdx.x] = s_rho; red[1][threadIdx.x] = s_ke; red[2][threadIdx.x] = mn; red[3][threadIdx.x] = mx;
-    red[4][threadIdx.x] = nonfin; red[5][threadIdx.x] = nfl; red[6][threadIdx.x] = s_f;
+    red[4][threadIdx.x] = nonfin; red[5][threadIdx.x] = nfl; red[6][threadIdx.x] = s_f; red[7][threadIdx.x] = vmax;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
@@ -1845,10 +1903,11 @@ __global__ void __launch_bounds__(256) stats_kernel(const void* __restrict__ f, 
             red[4][threadIdx.x] += red[4][threadIdx.x + s];
             red[5][threadIdx.x] += red[5][threadIdx.x + s];
             red[6][threadIdx.x] += red[6][threadIdx.x + s];
+            red[7][threadIdx.x] = fmax(red[7][threadIdx.x], red[7][threadIdx.x + s]);
         }
         __syncthreads();
     }
-    if (threadIdx.x < 7) partials[(size_t)blockIdx.x * 7 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 8) partials[(size_t)blockIdx.x * 8 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 hipError_t stats(const void* f, const float* rho, const float* vel, const uint8_t* obst, int32_t nb, int storage,
@@ -1858,6 +1917,30 @@ hipError_t stats(const void* f, const float* rho, const float* vel, const uint8_
     if (storage == 1) hipLaunchKernelGGL(stats_kernel<1>, grid, dim3(256), 0, s, f, rho, vel, obst, nb, partials);
     else if (storage == 2) hipLaunchKernelGGL(stats_kernel<2>, grid, dim3(256), 0, s, f, rho, vel, obst, nb, partials);
     else hipLaunchKernelGGL(stats_kernel<0>, grid, dim3(256), 0, s, f, rho, vel, obst, nb, partials);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Divergence check: non-finite density and velocity values per case.  Row y
+// of the grid is one (case, level) entry; its work-groups stride over the
+// entry's n cells (rho) and 3n velocity components and add their count to
+// counts[entry.slot].
+__global__ void __launch_bounds__(256) finite_kernel(const FiniteEntry* __restrict__ ents, int32_t* counts) {
+    const FiniteEntry e = ents[blockIdx.y];
+    int32_t bad = 0;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < e.n; i += stride) bad += isfinite(e.rho[i]) ? 0 : 1;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < 3 * e.n; i += stride)
+        bad += isfinite(e.vel[i]) ? 0 : 1;
+    for (int off = 32; off > 0; off >>= 1) bad += __shfl_down(bad, off, 64);
+    if ((threadIdx.x & 63) == 0 && bad != 0) atomicAdd(&counts[e.slot], bad);
+}
+
+hipError_t finite_count(const FiniteEntry* ents, int32_t n_ents, int32_t groups_per_entry, int32_t* counts,
+                        hipStream_t s) {
+    if (n_ents <= 0) return hipSuccess;
+    hipLaunchKernelGGL(finite_kernel, dim3((unsigned)groups_per_entry, (unsigned)n_ents), dim3(256), 0, s, ents,
+                       counts);
     return hipGetLastError();
 }
 

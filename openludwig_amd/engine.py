@@ -16,7 +16,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from .solver import LudwigError, Solver, ramp_velocity
+from .solver import LudwigError, Solver, averaged_coefficients, ramp_velocity
 
 
 def ramp_table(cfg, n_steps: int) -> np.ndarray:
@@ -85,6 +85,35 @@ class Engine:
 
     def done(self, solver: Solver) -> bool:
         return self.progress(solver) >= self.cases[solver]
+
+    def sample_forces(self, solver: Solver, diag_freq: int, ramp_steps: int | None = None) -> None:
+        """Sample the case's forces inside the engine step at every coarse
+        step t with t % diag_freq == 0 and t > ramp_steps (default: the
+        case's ramp), as Solver.run does at its diagnostic steps."""
+        ramp = solver.cfg.ramp_steps if ramp_steps is None else ramp_steps
+        self._check(self.lib.lb_engine_sample_forces(self._e, solver._ctx, C.byref(solver._force_params()),
+                                                     int(diag_freq), int(ramp)), "lb_engine_sample_forces")
+
+    def case_forces(self, solver: Solver) -> dict:
+        """The averaged coefficients of the case's samples so far
+        ({"n_samples", "Cd_avg", "Cl_avg"}, as Solver.run returns them)."""
+        sums = np.zeros(9)
+        n = C.c_int64()
+        self._check(self.lib.lb_engine_case_forces(self._e, solver._ctx, N.ptr(sums, C.c_double), C.byref(n)),
+                    "lb_engine_case_forces")
+        return averaged_coefficients(solver.params, sums, n.value)
+
+    def check_finite(self) -> list:
+        """Divergence check over every attached case (waits for the enqueued
+        work): the solvers whose density or velocity hold non-finite values."""
+        cap = max(1, len(self.cases))
+        arr = (C.c_void_p * cap)()
+        n = C.c_int32()
+        rc = self.lib.lb_engine_check_finite(self._e, arr, cap, C.byref(n))
+        if rc not in (0, N.LB_ENONFINITE):
+            self._check(rc, "lb_engine_check_finite")
+        bad = {arr[i] for i in range(min(n.value, cap))}
+        return [s for s in self.cases if s._ctx.value in bad]
 
     def set_options(self, graph: bool = True, schedule: int = N.LB_SCHED_XCD, waves: int = 5) -> None:
         self._check(self.lib.lb_engine_set_options(self._e, int(graph), int(schedule), int(waves)),

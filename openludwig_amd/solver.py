@@ -27,6 +27,14 @@ class LudwigError(RuntimeError):
     pass
 
 
+class DivergenceError(LudwigError):
+    """A case holds non-finite density or velocity values (LB_ENONFINITE)."""
+
+    def __init__(self, msg: str, count: int = 0):
+        super().__init__(msg)
+        self.count = count
+
+
 def step_params(cfg: CaseConfig, params: DomainParameters | None = None) -> N.LbStepParams:
     """Scalars of the launcher (timestep.jl:40-86) from the case configuration."""
     p = N.LbStepParams()
@@ -108,7 +116,21 @@ class ForceAccumulator:
         self.n_samples += 1
 
 
-def lb_config(cfg: CaseConfig, grids: list, params: DomainParameters, storage: int, strict_fp: bool) -> N.LbConfig:
+def averaged_coefficients(params: DomainParameters, sums: np.ndarray, n_samples: int) -> dict:
+    """Time-averaged coefficients from accumulated force samples
+    (accumulate_forces!, surface.jl:36-56): sums of Fp xyz, Fv xyz, M xyz."""
+    res = {"n_samples": int(n_samples)}
+    if n_samples:
+        q_inf = 0.5 * params.rho_physical * params.u_physical ** 2
+        F_ref = q_inf * params.reference_area
+        avg = np.asarray(sums, dtype=np.float64) / n_samples
+        res["Cd_avg"] = (avg[0] + avg[3]) / F_ref
+        res["Cl_avg"] = (avg[2] + avg[5]) / F_ref
+    return res
+
+
+def lb_config(cfg: CaseConfig, grids: list, params: DomainParameters, storage: int, strict_fp: bool,
+              wide_offsets: bool = False) -> N.LbConfig:
     c = N.LbConfig()
     c.n_levels = len(grids)
     c.domain_nx, c.domain_ny, c.domain_nz = params.nx_coarse, params.ny_coarse, params.nz_coarse
@@ -117,6 +139,7 @@ def lb_config(cfg: CaseConfig, grids: list, params: DomainParameters, storage: i
     c.q_min_threshold = cfg.q_min_threshold
     c.storage = storage
     c.strict_fp = int(strict_fp)
+    c.wide_offsets = int(wide_offsets)
     return c
 
 
@@ -137,7 +160,8 @@ def footprint(cfg: CaseConfig, grids: list, params: DomainParameters, mesh: Solv
 
 class Solver:
     def __init__(self, cfg: CaseConfig, grids: list, params: DomainParameters, mesh: SolverMesh | None = None,
-                 device: int = 0, strict_fp: bool = False, storage: int = N.LB_STORAGE_FP32):
+                 device: int = 0, strict_fp: bool = False, storage: int = N.LB_STORAGE_FP32,
+                 wide_offsets: bool = False):
         self.cfg = cfg
         self.grids = grids
         self.params = params
@@ -145,7 +169,7 @@ class Solver:
         self.lib = N.hip_lib()
         self._ctx = C.c_void_p()
         self.engine = None  # set while attached to a batched Engine
-        c = lb_config(cfg, grids, params, storage, strict_fp)
+        c = lb_config(cfg, grids, params, storage, strict_fp, wide_offsets)
         rc = self.lib.lb_create(C.byref(self._ctx), device, C.byref(c))
         if rc != 0:
             raise LudwigError(f"lb_create failed ({rc})")
@@ -286,10 +310,30 @@ class Solver:
         out = np.zeros(8)
         self._check(self.lib.lb_stats(self._ctx, level_id, N.ptr(out, C.c_double)), "lb_stats")
         return {"rho_sum": out[0], "kinetic_energy": out[1], "rho_min": out[2], "rho_max": out[3],
-                "nonfinite": int(out[4]), "fluid_cells": int(out[5]), "mass": out[6]}
+                "nonfinite": int(out[4]), "fluid_cells": int(out[5]), "mass": out[6], "v_max": out[7]}
+
+    def check_finite(self) -> None:
+        """Divergence check (lb_check_finite): raises DivergenceError when a
+        level holds non-finite density or velocity values."""
+        n = C.c_int64()
+        rc = self.lib.lb_check_finite(self._ctx, C.byref(n))
+        if rc == N.LB_ENONFINITE:
+            raise DivergenceError(f"case diverged: {self.lib.lb_last_error(self._ctx).decode(errors='replace')}",
+                                  int(n.value))
+        self._check(rc, "lb_check_finite")
+
+    def mass(self) -> float:
+        """FP64 sum of every population of every level, each level weighted by
+        its cell volume relative to the coarsest (the global mass check)."""
+        return sum(self.stats(g.level_id)["mass"] / 8.0 ** (g.level_id - 1) for g in self.grids)
 
     # -- main loop (main.jl:208-315) ----------------------------------------------
-    def run(self, steps: int, diag_freq: int | None = None, on_diag=None) -> dict:
+    def run(self, steps: int, diag_freq: int | None = None, on_diag=None, check_stability: bool = True) -> dict:
+        """The main loop of main.jl:208-315: batches of gpu_async_depth coarse
+        steps with the ramped inlet velocity; at every diagnostic step the
+        divergence check (DivergenceError on non-finite values) and the
+        forces, averaged over the samples after the ramp
+        (accumulate_forces!, surface.jl:36-56)."""
         cfg = self.cfg
         batch = cfg.gpu_async_depth
         diag = diag_freq or cfg.diag_freq
@@ -304,6 +348,8 @@ class Solver:
             if batch_end % diag < actual or batch_end == steps:
                 diag_step = (batch_end // diag) * diag
                 if t <= diag_step <= batch_end:
+                    if check_stability:
+                        self.check_finite()
                     rec = {"step": diag_step, "u_lat": u_curr}
                     if self.mesh is not None and cfg.forces_enabled:
                         f = self.compute_aerodynamics()
@@ -315,14 +361,8 @@ class Solver:
                         on_diag(rec)
             t = batch_end + 1
         self.synchronize()
-        res = {"history": history, "n_samples": acc.n_samples}
-        if acc.n_samples:
-            p = self.params
-            q_inf = 0.5 * p.rho_physical * p.u_physical ** 2
-            F_ref = q_inf * p.reference_area
-            avg = acc.sums / acc.n_samples
-            res["Cd_avg"] = (avg[0] + avg[3]) / F_ref
-            res["Cl_avg"] = (avg[2] + avg[5]) / F_ref
+        res = {"history": history}
+        res.update(averaged_coefficients(self.params, acc.sums, acc.n_samples))
         return res
 
 
