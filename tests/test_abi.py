@@ -54,7 +54,9 @@ def test_error_paths_without_device():
 
 def test_footprint_plans_without_gpu():
     """lb_footprint runs the upload code in counting mode: no device needed,
-    FP16 storage needs less than FP32, and the count grows with the mesh."""
+    FP16 storage needs less than FP32, and the count grows with the mesh.
+    Attaching to an engine adds the ramp table (4 bytes per batch) and the
+    80-byte force-sample accumulator."""
     from openludwig_amd import _native as N
     from openludwig_amd import cases
     from openludwig_amd.solver import build_case, footprint
@@ -65,4 +67,4 @@ def test_footprint_plans_without_gpu():
     n = sum(g.n_cells for g in grids)
     assert 216 * n < b32 < 400 * n
     assert 108 * n < b16 < b32
-    assert footprint(cfg, grids, params, mesh, N.LB_STORAGE_FP16, 10) == b16 + 40
+    assert footprint(cfg, grids, params, mesh, N.LB_STORAGE_FP16, 10) == b16 + 40 + 80
