@@ -1,6 +1,8 @@
 """Continuous-batching serving: requests that arrive while others run are
-admitted as memory frees up, share the engine, and each result equals the
-same request served alone (bit for bit: same Cd and Cl)."""
+admitted as memory frees up, share the engine, and each result -- the Cd and
+Cl averaged over the post-ramp force samples, taken inside the engine step --
+equals the same request served alone and served by the sequential loop
+(Solver.run's samples at its diagnostic steps), bit for bit."""
 import pytest
 
 from openludwig_amd import _native as N
@@ -37,5 +39,6 @@ def test_serving_batched_equals_alone_and_sequential(hip_device):
     seq = S.run_sequential(reqs, storage=N.LB_STORAGE_FP16)
     for a, b in zip(seq, mixed):
         assert (a.cd, a.cl) == (b.cd, b.cl), f"request {a.rid}: sequential {a.cd, a.cl} batched {b.cd, b.cl}"
+    assert all(r.n_samples >= 1 and r.status == "ok" for r in mixed)
     summ = S.summary(mixed)
     assert summ["n"] == 4 and summ["p99_s"] >= summ["p50_s"] > 0

@@ -257,16 +257,29 @@ def test_sponge_blends_rho_u_only():
 
 
 def test_positivity_limiter_clamps_negative():
+    """A strong diagonal non-equilibrium drives the unlimited output negative;
+    the positivity limiter (kernel_cumulant.jl:525-559) replaces it by
+    feq + alpha (f - feq) with one alpha for all 27 directions,
+    alpha = min over f - feq < 0 of feq / (feq - (f - feq) + 1e-15): every
+    population becomes positive, mass and momentum are kept."""
     L = lib("f64")
     w = weights(L)[3]
     f = w.copy()
-    f[0] += 0.004; f[26] -= 0.004                      # strong non-equilibrium along a diagonal
+    f[0] += 0.02; f[26] -= 0.02                        # strong non-equilibrium along a diagonal
     out0, _ = collide(L, f, 0.5002, params(limiter=0, omega_3=1.99, omega_4=1.99))
     out2, mac = collide(L, f, 0.5002, params(limiter=2, omega_3=1.99, omega_4=1.99))
-    if out0.min() < 0:
-        assert out2.min() >= -1e-15
-        # f = feq + alpha (f - feq) keeps mass
-        assert abs(out2.sum() - out0.sum()) < 1e-14
+    assert out0.min() < -1e-3                          # the limiter has something to do
+    assert out2.min() > 0.0
+    rho = f.sum()
+    u = (f[:, None] * CVEC).sum(0) / rho
+    feq = np.array([L.orc_feq(0, rho, *u, k) for k in range(27)])
+    fneq = out0 - feq
+    neg = (fneq < 0) & (feq > 1e-15)
+    alpha = max(min(1.0, float(np.min(feq[neg] / (feq[neg] - fneq[neg] + 1e-15)))), 0.0)
+    assert 0.0 < alpha < 1.0
+    assert np.abs(out2 - (feq + alpha * fneq)).max() < 1e-15
+    assert abs(out2.sum() - out0.sum()) < 1e-14
+    assert np.abs(((out2 - out0)[:, None] * CVEC).sum(0)).max() < 1e-14
 
 
 def test_wall_model_force_direction():
