@@ -418,3 +418,9 @@ def test_wall_stress_of_linear_shear_layer():
     assert abs(out[1] - (-(rho - 1.0) / 3.0 * ps * A.sum())) < 1e-5 * abs(out[1])      # Fy pressure = -p n_y A
     assert abs(out[3] - rho * nu * gam * ss * A.sum()) < 1e-5 * abs(out[3])              # Fx viscous
     assert out[0] == 0.0 and out[2] == 0.0 and out[5] == 0.0
+    # moments about the reference point: M = sum r x dF, dF = (tau_x A, -p A, 0), r = (x - 8, 4, z - 8)
+    pA = (rho - 1.0) / 3.0 * ps * A
+    tA = rho * nu * gam * ss * A
+    rx, rz = cx.astype(np.float64) - 8.0, cz.astype(np.float64) - 8.0
+    for got, exp in ((out[6], np.sum(rz * pA)), (out[7], np.sum(rz * tA)), (out[8], np.sum(-rx * pA - 4.0 * tA))):
+        assert abs(got - exp) <= 1e-5 * np.sum(np.abs(pA) + np.abs(tA)) * 8, (got, exp)

@@ -256,6 +256,29 @@ def test_sponge_blends_rho_u_only():
     assert abs(out.sum() - rho) < 1e-14
 
 
+def test_sponge_blend_distributions_moves_mass():
+    """sponge_blend_distributions = 1 (kernel_cumulant.jl:204-212) blends the
+    streamed populations toward feq(1, u_in) before the collision, which
+    keeps their sum: the output mass is (1 - sp) rho + sp (without the blend
+    the collision returns rho itself)."""
+    L = lib("f64")
+    rng = np.random.default_rng(19)
+    w = weights(L)[3]
+    f = w * (1.04 + 0.05 * rng.uniform(-1, 1, 27))
+    sp, u_in = float(np.float32(0.3)), float(np.float32(0.05))
+    rho = f.sum()
+    plain, _ = collide(L, f, 0.6, params(), sponge=sp, u_in=u_in)
+    blend, _ = collide(L, f, 0.6, params(sponge_blend=1), sponge=sp, u_in=u_in)
+    assert abs(plain.sum() - rho) < 1e-14
+    assert abs(blend.sum() - ((1 - sp) * rho + sp)) < 1e-14
+    # the momentum is the blended mass times the blended velocity (the collision is taken around
+    # u = (1 - sp) j / rho + sp u_in x, with zero first central moments)
+    u = (f[:, None] * CVEC).sum(0) / rho
+    ub = (1 - sp) * u + np.array([sp * u_in, 0.0, 0.0])
+    jb = (blend[:, None] * CVEC).sum(0)
+    assert np.abs(jb - ((1 - sp) * rho + sp) * ub).max() < 1e-14
+
+
 def test_positivity_limiter_clamps_negative():
     """A strong diagonal non-equilibrium drives the unlimited output negative;
     the positivity limiter (kernel_cumulant.jl:525-559) replaces it by
