@@ -12,6 +12,13 @@ extruded NACA 0012 wing at alpha 2 deg with the thermal DDF, 29.4 M cells.
 
 A "step" is one coarse time step: the full multilevel recursion (15
 substeps over 4 levels), issued in batches of 8 like execute_timestep_batch!.
+The run is the workload's stated length (cube: 2,500 coarse steps = 20,000
+finest steps) with an inflow ramp of --ramp coarse steps (500): a spin-up
+brings the flow to coarse step total - warmup - steps, then the warm-up
+steps and the timed steps follow, so the timed window is the developed flow
+at the end of the run.  Forces are sampled every --diag coarse steps after
+the ramp, as the main loop does (main.jl:236-243), outside the timed window;
+Cd/Cl are their averages (accumulate_forces!, surface.jl:36-56).
 value = MLUPS_true (every substep's cell updates counted) over the timed
 steps, with the populations resident in HBM.  roofline = the finest level's
 stream-collide kernel (the dominant one): algorithmic bytes per launch
@@ -23,7 +30,8 @@ average launch time from HIP events on the library's stream.
 cpu_baseline = the C oracle (the restated reference algorithm) on the host
 cores over one coarse step of the same workload.  serving = the serving
 comparison of openludwig_amd.serving.compare (FP16 continuous batching vs
-plain FP32 sequential) on a small request set.
+plain FP32 sequential) on the serving request set: 64 requests of 1-40 M
+cells, 2-5 levels.
 """
 from __future__ import annotations
 
@@ -53,6 +61,17 @@ STORAGE_CODE = {"fp32": 0, "fp16": 1, "bf16": 2}
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -71,8 +90,15 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=1, help="coarse steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--probe-gib", type=float, default=2.0,
                     help="buffer size of the HBM bandwidth probe (0: skip)")
-    ap.add_argument("--serving-requests", type=int, default=8,
+    ap.add_argument("--serving-requests", type=int, default=64,
                     help="requests in the serving comparison, FP16 batched vs plain FP32 (0: skip)")
+    ap.add_argument("--serving-cells", type=float, nargs=2, default=(1e6, 40e6),
+                    help="active-cell range of the serving requests")
+    ap.add_argument("--total", type=int, default=-1,
+                    help="coarse steps of the whole run (-1: the workload's stated length; the spin-up is "
+                         "total - warmup - steps, 0 or less: none)")
+    ap.add_argument("--ramp", type=int, default=500, help="inflow ramp, coarse steps")
+    ap.add_argument("--diag", type=int, default=100, help="force sampling cadence after the ramp, coarse steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU form of this "
@@ -81,7 +107,8 @@ def main() -> int:
         return 2
 
     from openludwig_amd import cases
-    from openludwig_amd.solver import Solver, build_case, mlups, ramp_velocity
+    from openludwig_amd.solver import (DivergenceError, ForceAccumulator, Solver, averaged_coefficients, build_case,
+                                       mlups, ramp_velocity)
 
     t0 = time.time()
     if args.workload == "cube":
@@ -98,24 +125,41 @@ def main() -> int:
     log(f"setup {time.time() - t0:.1f}s: levels {len(grids)}, cells {[f'{c / 1e6:.2f}M' for c in n_cells]}, "
         f"bouzidi cells {grids[-1].n_boundary_cells}, tau {params.tau_levels}")
 
+    total = cfg.steps if args.total < 0 else args.total
+    total = max(total, args.warmup + args.steps)
+    spinup = total - args.warmup - args.steps
+    cfg = cfg.replace(ramp_steps=args.ramp, steps=total)
     solver = Solver(cfg, grids, params, mesh, strict_fp=args.strict, storage=STORAGE_CODE[args.storage])
     b_alg = B_ALG_WMLES[args.storage]
     solver.init_equilibrium()
+    mass0 = solver.mass()
     batch = cfg.gpu_async_depth
+    acc = ForceAccumulator()
 
-    def run(t_first: int, n: int) -> None:
+    def run(t_first: int, n: int, sample: bool) -> None:
+        """Coarse steps t_first .. t_first+n-1 in batches of gpu_async_depth
+        (inlet velocity at each batch end, main.jl:214-218); with `sample`,
+        the forces after every batch that holds a multiple of the sampling
+        cadence past the ramp (main.jl:226-243) and the divergence check."""
         t = t_first
         end = t_first + n - 1
         while t <= end:
             be = min(t + batch - 1, end)
             solver.execute_timestep_batch(t, be - t + 1, ramp_velocity(cfg.u_lattice, be, cfg.ramp_steps))
+            d = (be // args.diag) * args.diag
+            if sample and t <= d <= be and d > cfg.ramp_steps:
+                solver.check_finite()
+                acc.add(solver.compute_aerodynamics())
             t = be + 1
 
     def sync() -> None:
         solver.synchronize()
 
-    run(1, args.warmup)
+    ts = time.perf_counter()
+    run(1, spinup + args.warmup, True)
     sync()
+    t_spin = time.perf_counter() - ts
+    log(f"spin-up and warm-up: {spinup + args.warmup} coarse steps in {t_spin:.1f}s, {acc.n_samples} force samples")
     finest = grids[-1].level_id
     solver.profile(True)
     for g in grids:
@@ -123,17 +167,30 @@ def main() -> int:
         solver.profile_read(g.level_id, 1)
     sync()
     t_start = time.perf_counter()
-    run(1 + args.warmup, args.steps)
+    run(1 + spinup + args.warmup, args.steps, False)
     sync()
     elapsed = time.perf_counter() - t_start
     kern = {g.level_id: solver.profile_read(g.level_id) for g in grids}
     kern_t = {g.level_id: solver.profile_read(g.level_id, 1) for g in grids}
     solver.profile(False)
-    st = solver.stats(1)
-    if st["nonfinite"]:
-        log(f"non-finite populations detected: {st['nonfinite']}")
+    try:
+        solver.check_finite()
+    except DivergenceError as exc:
+        log(str(exc))
         return 3
+    t_end = spinup + args.warmup + args.steps
+    # the sample the timed window holds (its last step is a multiple of the cadence by construction)
+    d_last = (t_end // args.diag) * args.diag
+    if d_last > spinup + args.warmup and d_last > cfg.ramp_steps:
+        acc.add(solver.compute_aerodynamics())
     forces = solver.compute_aerodynamics()
+    coeffs = averaged_coefficients(params, acc.sums, acc.n_samples)
+    lstats = [solver.stats(g.level_id) for g in grids]
+    mass1 = solver.mass()
+    diagnostics = {"rho_min": round(min(s_["rho_min"] for s_ in lstats), 6),
+                   "rho_max": round(max(s_["rho_max"] for s_ in lstats), 6),
+                   "v_max": round(max(s_["v_max"] for s_ in lstats), 6),
+                   "mass_rel_change": float(f"{(mass1 - mass0) / mass0:.4g}")}
 
     # HBM bytes per launch of the same kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes (scaled by a copy of known size)
@@ -165,7 +222,8 @@ def main() -> int:
     step_bytes = sum(g.n_cells * 2 ** (g.level_id - 1) * b_lev[g.level_id] for g in grids)
     step_lu = sum(g.n_cells * 2 ** (g.level_id - 1) for g in grids)
     log(f"timed {args.steps} coarse steps in {elapsed:.3f}s: MLUPS_true {m_true:.0f}, MLUPS_ref {m_ref:.0f}; "
-        f"finest collide {avg_ms:.3f} ms/launch = {achieved:.0f} GB/s; Cd {forces['Cd']:.4f}")
+        f"finest collide {avg_ms:.3f} ms/launch = {achieved:.0f} GB/s; Cd avg {coeffs.get('Cd_avg', float('nan')):.4f} "
+        f"over {acc.n_samples} samples, Cd at the end {forces['Cd']:.4f}")
 
     # practical HBM ceiling of plain streaming kernels on this device
     probe = None
@@ -199,7 +257,8 @@ def main() -> int:
                "sample": f"{args.cpu_steps} coarse step(s) of the same {sum(n_cells) / 1e6:.1f}M-cell, "
                          f"{len(grids)}-level workload "
                          f"({sum(c * 2 ** i for i, c in enumerate(n_cells)) * args.cpu_steps / 1e6:.0f}M cell "
-                         f"updates), C oracle (FP32 mirror, OpenMP)", "seconds": round(tcpu, 2)}
+                         f"updates), C oracle (FP32 mirror, OpenMP)", "seconds": round(tcpu, 2),
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
         log(f"cpu baseline {c_true:.1f} MLUPS on {threads} threads ({tcpu:.1f}s)")
 
     serving = None
@@ -207,7 +266,7 @@ def main() -> int:
         from openludwig_amd import serving as S
         if "solver" in locals():
             del solver
-        serving = S.compare(args.serving_requests, log=log)
+        serving = S.compare(args.serving_requests, cells=tuple(args.serving_cells), log=log)
 
     line = {
         "metric": "MLUPS & % HBM roofline; served cases/h + p99 latency vs plain FP32",
@@ -220,7 +279,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32" if args.storage == "fp32" else f"{args.storage}-storage/fp32-compute",
         "data": "synthetic",
         "config": {"workload": f"{wname}, {len(grids)} levels, {sum(n_cells) / 1e6:.1f}M cells, "
                                f"resolution {args.resolution}",
@@ -229,10 +288,13 @@ def main() -> int:
                    "population_storage": args.storage, "bytes_per_lu": round(step_bytes / step_lu, 2),
                    "hbm_roofline_pct_step": round(m_true * 1e6 * step_bytes / step_lu / (HBM_PEAK_GBS * 1e9) * 100, 2),
                    "per_level_collide": per_level,
-                   # a sanity figure early in the inflow ramp (coarse step warmup + steps of
-                   # ramp_steps), normalised by the full inflow velocity: small by design
-                   "Cd": float(f"{forces['Cd']:.6g}"), "Cd_at_coarse_step": args.warmup + args.steps,
-                   "ramp_steps": cfg.ramp_steps, "thermal": thermal},
+                   "coarse_steps_total": t_end, "timed_window": [t_end - args.steps + 1, t_end],
+                   "ramp_steps": cfg.ramp_steps, "force_samples_every": args.diag,
+                   "Cd_avg": float(f"{coeffs['Cd_avg']:.6g}") if acc.n_samples else None,
+                   "Cl_avg": float(f"{coeffs['Cl_avg']:.6g}") if acc.n_samples else None,
+                   "force_samples": acc.n_samples,
+                   "Cd_end": float(f"{forces['Cd']:.6g}"), "Cl_end": float(f"{forces['Cl']:.6g}"),
+                   "diagnostics_fp64": diagnostics, "thermal": thermal},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "collide_kernel (finest level)", "bytes_per_launch": bytes_launch,

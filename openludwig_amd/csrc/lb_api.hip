@@ -803,6 +803,8 @@ hipError_t engine_issue_step(lb_engine* e) {
     return e->k->advance(e->states, (int32_t)e->cases.size(), e->stream);
 }
 
+constexpr int kStatsRows = 1024;  // the statistics kernel runs at most this many work-groups
+
 // Divergence check entry of one level: its density and velocity in buffer
 // `cur` (the output of the last completed coarse step).
 constexpr int32_t kFiniteGroups = 64;  // work-groups per (case, level) entry
@@ -1388,10 +1390,10 @@ int lb_stats(lb_ctx* c, int level_id, double* out) {
     if (!out || level_id < 1 || level_id > (int)c->levels.size()) return fail(c, LB_EINVAL, "bad arguments");
     HIPCHK(c, hipSetDevice(c->device));
     const Level& L = c->levels[(size_t)level_id - 1];
-    const int groups = std::max(1, std::min<int>(L.nb, 1024));
+    const int groups = std::max(1, std::min<int>(L.nb, kStatsRows));
     if (c->n_stat_groups < groups) {
-        RC(dalloc(c, &c->spart, (size_t)1024 * 8));
-        c->n_stat_groups = 1024;
+        RC(dalloc(c, &c->spart, (size_t)kStatsRows * 8));
+        c->n_stat_groups = kStatsRows;
     }
     for (int i = 0; i < 8; i++) out[i] = 0;
     if (L.nb == 0) return LB_OK;

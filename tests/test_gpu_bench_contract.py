@@ -2,7 +2,9 @@
 contract names, a roofline object whose fraction is its achieved rate over
 the HBM peak, and the CPU baseline of the oracle.  The GPU test runs the
 cube workload at a small resolution (a few million cells, 2 timed coarse
-steps) in a child process; the CPU test checks the refusal of N > 1."""
+steps) in a child process: FP32 after a short spin-up with force samples,
+and FP16 storage (its dtype names the 16-bit storage); the CPU test checks
+the refusal of N > 1."""
 from __future__ import annotations
 
 import json
@@ -24,10 +26,12 @@ def test_bench_refuses_more_than_one_gpu():
 
 
 @pytest.mark.gpu
-def test_bench_json_line_contract():
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_bench_json_line_contract(storage):
+    extra = ["--total", "40", "--ramp", "8", "--diag", "8"] if storage == "fp32" else ["--total", "0"]
     r = subprocess.run([sys.executable, "bench.py", "--resolution", "48", "--steps", "2", "--warmup", "1",
-                        "--cpu-steps", "1", "--serving-requests", "0", "--probe-gib", "0"],
-                       cwd=ROOT, capture_output=True, text=True, timeout=300)
+                        "--cpu-steps", "1", "--serving-requests", "0", "--probe-gib", "0", "--storage", storage]
+                       + extra, cwd=ROOT, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-3000:]
     lines = [ln for ln in r.stdout.splitlines() if ln.strip()]
     assert len(lines) == 1, r.stdout
@@ -37,7 +41,15 @@ def test_bench_json_line_contract():
         assert key in d, key
     assert d["n_gpus"] == 1 and d["steps"] == 2 and d["warmup"] == 1
     assert d["higher_is_better"] is True and d["unit"] == "MLUPS" and d["value"] > 0
-    assert d["dtype"] == "fp32" and "workload" in d["config"]
+    assert d["dtype"] == ("fp32" if storage == "fp32" else "fp16-storage/fp32-compute")
+    assert "workload" in d["config"] and d["config"]["population_storage"] == storage
+    c = d["config"]
+    if storage == "fp32":
+        # coarse steps 1..40: samples after the ramp at 16, 24, 32 (spin-up) and 40 (end of the timed window)
+        assert c["timed_window"] == [39, 40] and c["force_samples"] == 4 and c["Cd_avg"] is not None
+    else:
+        assert c["timed_window"] == [2, 3] and c["force_samples"] == 0
+    assert c["diagnostics_fp64"]["v_max"] > 0.0
     assert abs(d["ms_per_step"] - 1e3 * d["config"]["lu_per_step"] / (d["value"] * 1e6)) <= 0.01 * d["ms_per_step"]
     rf = d["roofline"]
     assert rf["bound"] == "hbm" and rf["unit"] == "GB/s" and rf["peak"] == 8000.0
@@ -46,3 +58,4 @@ def test_bench_json_line_contract():
     assert abs(rf["achieved"] - rf["bytes_per_launch"] / (rf["avg_launch_ms"] * 1e-3) / 1e9) <= 0.01 * rf["achieved"]
     cb = d["cpu_baseline"]
     assert cb["kind"] == "port" and cb["value"] > 0 and cb["cores"] >= 1 and cb["unit"] == "MLUPS"
+    assert cb["cpu_model"] != "unknown"

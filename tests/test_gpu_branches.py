@@ -72,7 +72,7 @@ def test_isothermal_wall_strict_bitexact(hip_device, storage):
     # solid cells carry T_wall; the fluid next to the wall warms up
     g = grids[-1]
     T = s.download(g.level_id, "T")
-    solid = g.obstacle.astype(bool)
+    solid = g.obstacle.ravel().astype(bool)
     assert np.all(T[solid] == np.float32(1.1))
     assert T[~solid].max() > 1.0 + 1e-4
 
