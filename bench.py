@@ -235,10 +235,10 @@ def main() -> int:
         rc = N.hip_lib().lb_bandwidth_probe(0, int(args.probe_gib * (1 << 30)), 10, N.ptr(pb, C.c_double))
         if rc == 0:
             probe = {"copy": round(pb[0], 1), "read": round(pb[1], 1), "write": round(pb[2], 1),
-                     "copy_nontemporal": round(pb[3], 1), "copy_nontemporal_x4": round(pb[4], 1),
+                     "copy_nontemporal": round(pb[3], 1), "copy_nontemporal_grid_stride": round(pb[4], 1),
                      "unit": "GB/s", "buffer_gib": args.probe_gib}
             log(f"bandwidth probe: copy {pb[0]:.0f}, read {pb[1]:.0f}, write {pb[2]:.0f}, "
-                f"copy NT {pb[3]:.0f}, copy NT x4 {pb[4]:.0f} GB/s")
+                f"copy NT {pb[3]:.0f}, copy NT grid-stride {pb[4]:.0f} GB/s")
 
     cpu = None
     if args.cpu_steps > 0:

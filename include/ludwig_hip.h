@@ -329,11 +329,12 @@ void lb_free(void* p);
 const char* lb_setup_gpu_last_error(void);
 
 /* HBM bandwidth probe (no counterpart in the reference): streams `bytes`
- * with 16-byte accesses; gbps_out[0] copy (read + write counted),
- * [1] read-only, [2] write-only, [3] copy with non-temporal loads and
- * stores, [4] the same with four loads in flight per work-item (five
- * doubles), each averaged over `reps` launches.  The practical ceiling
- * reported beside the nominal 8 TB/s.                                       */
+ * with 16-byte accesses, one per work-item over a grid sized to the buffer;
+ * gbps_out[0] copy (read + write counted), [1] read-only, [2] write-only,
+ * [3] copy with non-temporal loads and stores, [4] the same copy as a
+ * grid-stride loop over 8 work-groups per CU (five doubles), each averaged
+ * over `reps` launches.  The practical ceiling reported beside the nominal
+ * 8 TB/s.                                                                  */
 int lb_bandwidth_probe(int device, int64_t bytes, int reps, double* gbps_out);
 
 #ifdef __cplusplus

@@ -58,7 +58,7 @@ def main() -> int:
         out = np.zeros(5)
         rc = N.hip_lib().lb_bandwidth_probe(0, int(args.probe_gib * (1 << 30)), 10, N.ptr(out, C.c_double))
         print(f"bandwidth probe rc={rc}: copy {out[0]:.0f} GB/s, read {out[1]:.0f} GB/s, write {out[2]:.0f} GB/s, "
-              f"copy non-temporal {out[3]:.0f} GB/s, copy non-temporal x4 {out[4]:.0f} GB/s", flush=True)
+              f"copy non-temporal {out[3]:.0f} GB/s, copy non-temporal grid-stride {out[4]:.0f} GB/s", flush=True)
     if args.copy_mib > 0:
         # one device-to-device copy through the HIP runtime (a blit kernel)
         import ctypes as C

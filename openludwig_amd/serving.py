@@ -318,6 +318,9 @@ def run_batched(requests: list, storage: int = N.LB_STORAGE_FP16, strict_fp: boo
     pending = sorted(requests, key=lambda r: r.arrival)
     active: dict = {}
     out = []
+    # where the loop's wall time goes: engine steps (enqueue + wait), the
+    # per-chunk checks and completions, tuning, and waiting with nothing admitted
+    tm = {"steps_s": 0.0, "checks_and_completions_s": 0.0, "tune_s": 0.0, "idle_s": 0.0, "chunks": 0}
     with ThreadPoolExecutor(max_workers=setup_workers) as pool:
         while len(out) < len(requests):
             now = time.perf_counter() - t0
@@ -333,13 +336,19 @@ def run_batched(requests: list, storage: int = N.LB_STORAGE_FP16, strict_fp: boo
                     active[p.solver] = p
             if active and not options:
                 key = (storage, max(len(p.grids) for p in active.values()))
+                ta = time.perf_counter()
                 if key not in _TUNED and tune:
                     _TUNED[key] = engine.autotune()["options"]
                 if key in _TUNED and engine.options != _TUNED[key]:
                     engine.set_options(**_TUNED[key])
+                tm["tune_s"] += time.perf_counter() - ta
             if active:
+                ta = time.perf_counter()
                 engine.step(chunk)
                 engine.synchronize()
+                tb = time.perf_counter()
+                tm["steps_s"] += tb - ta
+                tm["chunks"] += 1
                 diverged = set(engine.check_finite())
                 for s in [s for s in active if s in diverged or engine.done(s)]:
                     p = active.pop(s)
@@ -360,12 +369,16 @@ def run_batched(requests: list, storage: int = N.LB_STORAGE_FP16, strict_fp: boo
                     if log:
                         log(f"  batched: request {p.req.rid} {status} at {out[-1].finish:.1f}s "
                             f"({out[-1].cells / 1e6:.1f}M cells, {len(active)} still running)")
+                tm["checks_and_completions_s"] += time.perf_counter() - tb
             else:
+                ta = time.perf_counter()
                 nxt = pending[0].arrival - now if pending else 0.002
                 time.sleep(min(max(nxt, 0.0), 0.002))
+                tm["idle_s"] += time.perf_counter() - ta
     engine.close()
     if info is not None:
-        info.update({"budget_bytes": int(budget.limit), "peak_admitted_bytes": int(budget.peak)})
+        info.update({"budget_bytes": int(budget.limit), "peak_admitted_bytes": int(budget.peak),
+                     "loop": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tm.items()}})
     return sorted(out, key=lambda r: r.rid)
 
 
@@ -427,6 +440,7 @@ def compare(n: int = 64, cells: tuple = (1e6, 40e6), finest_steps: int = 1024, l
             "capacity_cases_per_hour": cap,
             "admission": {"budget_gb": round(info.get("budget_bytes", 0) / 1e9, 1),
                           "peak_admitted_gb": round(info.get("peak_admitted_bytes", 0) / 1e9, 2)},
+            "batched_closed_loop_time": info.get("loop"),
             "arrival_rate_per_s": round(load * mu, 4), "load_vs_plain_throughput": load,
             "plain_fp32_sequential": {k: round(v, 3) for k, v in ss.items()},
             "fp16_batched": {k: round(v, 3) for k, v in bs.items()},
