@@ -203,6 +203,16 @@ int lb_upload_level(lb_ctx* ctx, int level_id, int field, const float* src);
  * [7] v_max = max |u| over fluid cells.                                     */
 int lb_stats(lb_ctx* ctx, int level_id, double* out);
 
+/* lb_stats without allocating: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under stream capture, tight
+ * serving loops).  The same outputs as lb_stats, to the bit; lb_stats
+ * allocates its 64 KiB of partial sums on first use, this variant keeps them
+ * and the reduced result in a fixed per-device array of 65,600 bytes
+ * (1,024 partial rows of 8 doubles, the most the statistics kernel
+ * produces for any level, plus 8).  Calls that overlap on one GPU share that
+ * array and race; the caller serialises them.                             */
+int lb_stats_noalloc(lb_ctx* ctx, int level_id, double* out);
+
 /* Divergence check (check_stability, diagnostics.jl:95-121, which only
  * prints warnings; here a diverged case is an error): counts non-finite
  * density and velocity values of every level after the last completed

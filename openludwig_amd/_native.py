@@ -82,6 +82,7 @@ HIP_EXPORTS = {
     "lb_upload_level": (C.c_int, [C.c_void_p, C.c_int, C.c_int, f32p]),
     "lb_stats": (C.c_int, [C.c_void_p, C.c_int, f64p]),
     "lb_check_finite": (C.c_int, [C.c_void_p, i64p]),
+    "lb_stats_noalloc": (C.c_int, [C.c_void_p, C.c_int, f64p]),
     "lb_device_bytes": (C.c_int64, [C.c_void_p]),
     "lb_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "lb_profile_read": (C.c_int, [C.c_void_p, C.c_int, f64p, i64p]),

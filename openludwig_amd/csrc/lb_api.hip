@@ -803,8 +803,6 @@ hipError_t engine_issue_step(lb_engine* e) {
     return e->k->advance(e->states, (int32_t)e->cases.size(), e->stream);
 }
 
-constexpr int kStatsRows = 1024;  // the statistics kernel runs at most this many work-groups
-
 // Divergence check entry of one level: its density and velocity in buffer
 // `cur` (the output of the last completed coarse step).
 constexpr int32_t kFiniteGroups = 64;  // work-groups per (case, level) entry
@@ -826,6 +824,71 @@ int check_ctx(lb_ctx* c) {
     return LB_OK;
 }
 
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// lb_stats_noalloc: the statistics of lb_stats with their partial rows and
+// the reduced result in a fixed device array instead of memory allocated on
+// first use, for callers that cannot allocate.  One copy of the array per
+// device; its address on each device is taken once, at context creation on
+// that device (hipGetSymbolAddress), and only that address is handed to
+// kernels and copies.
+namespace {
+constexpr int kStatsRows = 1024;  // the statistics kernel runs at most this many work-groups
+constexpr size_t kScratchDoubles = (size_t)kStatsRows * 8 + 8;
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// Address of `scratch` on `device` (the current device), taken once.
+void scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) g_scratch_ptr[device] = (double*)p;
+    else {
+        g_scratch_err[device] = e;
+        (void)hipGetLastError();
+    }
+}
+
+double* scratch_on(int device, hipError_t* err) {
+    if (device < 0 || device >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    *err = g_scratch_err[device];
+    return g_scratch_ptr[device];
+}
+
+// The partial rows reduced in row order, with the operations lb_stats
+// applies on the host (so the results are the same to the bit).
+__global__ void stats_reduce_kernel(const double* __restrict__ part, int rows, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s0 = 0, s1 = 0, s4 = 0, s5 = 0, s6 = 0, mn = 1e300, mx = -1e300, vmax = 0;
+    for (int g = 0; g < rows; g++) {
+        const double* q = part + (size_t)g * 8;
+        s0 += q[0];
+        s1 += q[1];
+        mn = q[2] < mn ? q[2] : mn;
+        mx = mx < q[3] ? q[3] : mx;
+        s4 += q[4];
+        s5 += q[5];
+        s6 += q[6];
+        vmax = vmax < q[7] ? q[7] : vmax;
+    }
+    out[0] = s0; out[1] = s1; out[2] = mn; out[3] = mx;
+    out[4] = s4; out[5] = s5; out[6] = s6; out[7] = vmax;
+}
 }  // namespace
 
 namespace lbsync {
@@ -863,6 +926,7 @@ int lb_create(lb_ctx** out, int device, const lb_config* cfg) {
         delete c;
         return LB_EHIP;
     }
+    scratch_setup(device);
     c->levels.resize((size_t)cfg->n_levels);
     c->las_host.resize((size_t)cfg->n_levels);
     std::memset((void*)c->las_host.data(), 0, c->las_host.size() * sizeof(lbk::LevelArgs));
@@ -1438,6 +1502,31 @@ int lb_check_finite(lb_ctx* c, int64_t* n_nonfinite) {
     }
     if (n_nonfinite) *n_nonfinite = count;
     if (count > 0) return fail(c, LB_ENONFINITE, "%d non-finite density/velocity values", count);
+    return LB_OK;
+}
+
+int lb_stats_noalloc(lb_ctx* c, int level_id, double* out) {
+    lbsync::SetupGuard guard;
+    RC(check_ctx(c));
+    if (!out || level_id < 1 || level_id > (int)c->levels.size()) return fail(c, LB_EINVAL, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    const Level& L = c->levels[(size_t)level_id - 1];
+    const int groups = std::max(1, std::min<int>(L.nb, kStatsRows));
+    if (groups > kStatsRows)
+        return fail(c, LB_EINVAL, "lb_stats_noalloc: %d partial rows exceed the scratch of %d", groups, kStatsRows);
+    for (int i = 0; i < 8; i++) out[i] = 0;
+    if (L.nb == 0) return LB_OK;
+    hipError_t se = hipSuccess;
+    double* s = scratch_on(c->device, &se);
+    if (!s) return fail(c, LB_EHIP, "lb_stats_noalloc: no scratch address on device %d (%s)", c->device,
+                        hipGetErrorString(se));
+    HIPCHK(c, c->k->stats(L.f[L.cur], L.rho[L.cur], L.vel[L.cur], L.obst, L.nb, c->cfg.storage, s, groups,
+                          c->stream));
+    double* res = s + (size_t)kStatsRows * 8;
+    hipLaunchKernelGGL(stats_reduce_kernel, dim3(1), dim3(64), 0, c->stream, s, groups, res);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(out, res, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return LB_OK;
 }
 
