@@ -110,6 +110,9 @@ struct Level {
     lbk::Work* wlk = nullptr;
     int cur = 0;        // buffer index written by the last step (0 before any step)
     std::vector<int32_t> order_host;  // processing order (engine work lists)
+    // Device block i holds the caller's block perm[i] (0-based); the device
+    // order is the slab order of lb_level_upload.
+    std::vector<int32_t> perm;
     int32_t n_int = 0;                // interior blocks: the first n_int of the order
     // kernel timing: event pairs recorded around collide launches
     // [0] stream-collide launches, [1] thermal launches
@@ -195,7 +198,7 @@ struct lb_engine {
     int32_t n_active = 0;
     // options
     bool use_graph = true;
-    int sched = 0;
+    int sched = LB_SCHED_LINEAR;
     int waves = 5;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -357,9 +360,10 @@ int build_links(lb_ctx* c, Level& L, const lb_level_desc* d) {
             return fail(c, LB_EINVAL, "level %d: Bouzidi cell %lld out of range", d->level_id, (long long)i);
         int cl = (x - 1) + 8 * (y - 1) + 64 * (z - 1);
         for (int k = 0; k < Q; k++) {
-            uint16_t bits = d->bouzidi_q_cells
-                                ? d->bouzidi_q_cells[i * Q + k]
-                                : d->bouzidi_q_dense[(size_t)cl + 512 * (size_t)(b - 1) + 512 * (size_t)nb * k];
+            // the dense q map is in the caller's block order
+            const size_t bref = L.perm.empty() ? (size_t)(b - 1) : (size_t)L.perm[(size_t)b - 1];
+            uint16_t bits = d->bouzidi_q_cells ? d->bouzidi_q_cells[i * Q + k]
+                                               : d->bouzidi_q_dense[(size_t)cl + 512 * bref + 512 * (size_t)nb * k];
             float q = half_to_float(bits);
             if (!(q > qmin && q <= 1.0f)) continue;
             int ok = 26 - k;
@@ -473,18 +477,19 @@ int step_level(lb_ctx* c, int li, int64_t j) {
         if (fill) HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 0, c->stream));
         hipEvent_t e1 = nullptr;
         RC(profile_begin(c, L, 0, &e1));
-        HIPCHK(c, c->k->collide(las, st, L.wcol, L.n_int, j, storage, L.wide ? 1 : 0, 0, 0, 5, c->stream));
-        HIPCHK(c, c->k->collide(las, st, L.wcol + L.n_int, L.nb - L.n_int, j, storage, L.wide ? 1 : 0, 1, 0, 5,
-                                c->stream));
+        HIPCHK(c, c->k->collide(las, st, L.wcol, L.n_int, j, storage, L.wide ? 1 : 0, 0, LB_SCHED_LINEAR, 5, c->stream));
+        HIPCHK(c, c->k->collide(las, st, L.wcol + L.n_int, L.nb - L.n_int, j, storage, L.wide ? 1 : 0, 1,
+                                LB_SCHED_LINEAR, 5, c->stream));
         if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
         if (L.n_lchunks > 0) HIPCHK(c, c->k->bouzidi(las, st, L.wlk, L.n_lchunks, j, storage, 0, c->stream));
         if (ddf) {
             if (fill) HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 1, c->stream));
             hipEvent_t e2 = nullptr;
             RC(profile_begin(c, L, 1, &e2));
-            HIPCHK(c, c->k->thermal(las, st, L.wcol, L.n_int, j, storage, L.wide_g ? 1 : 0, 0, 0, c->stream));
-            HIPCHK(c, c->k->thermal(las, st, L.wcol + L.n_int, L.nb - L.n_int, j, storage, L.wide_g ? 1 : 0, 1, 0,
+            HIPCHK(c, c->k->thermal(las, st, L.wcol, L.n_int, j, storage, L.wide_g ? 1 : 0, 0, LB_SCHED_LINEAR,
                                     c->stream));
+            HIPCHK(c, c->k->thermal(las, st, L.wcol + L.n_int, L.nb - L.n_int, j, storage, L.wide_g ? 1 : 0, 1,
+                                    LB_SCHED_LINEAR, c->stream));
             if (e2) HIPCHK(c, hipEventRecord(e2, c->stream));
             if (L.n_lchunks > 0) HIPCHK(c, c->k->bouzidi(las, st, L.wlk, L.n_lchunks, j, storage, 1, c->stream));
         }
@@ -971,6 +976,72 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     if (d->ptr_dim_x < 0 || d->ptr_dim_y < 0 || d->ptr_dim_z < 0)
         return fail(c, LB_EINVAL, "level %d: bad block_pointer dims", d->level_id);
     if (!c->dry) HIPCHK(c, hipSetDevice(c->device));
+    // Device block order.  The caller's blocks come in the reference's
+    // lexicographic (bx, by, bz) order, z fastest.  On the device they are
+    // stored, and processed, in 4-layer z-slabs: inside a slab x-rows of
+    // 4-block z-columns, x fastest, then y.  The stream-collide pull reads a
+    // 4-byte value per cell of each x face of a neighbour block out of
+    // 128-byte lines that hold 4 rows of 8 cells (a y face reads one 32-byte
+    // run per line, a z face whole lines), so x neighbours should be read
+    // right after each other and y and z neighbours soon after, while their
+    // lines are still in L2 or the last-level cache; and processing in
+    // storage order streams the level through memory.  Measured on the
+    // 52.5M-cell cube case (MI355X, FP32), finest stream-collide: the
+    // caller's order with the slab work order dealt to XCDs in runs 1680 us;
+    // slab storage in order 1545 us (1612 us in XCD runs); x fastest in order
+    // 1904 us (the z neighbours fall out of the caches); the caller's order in
+    // order 1634 us.  A 2-D Morton order of the columns in place of the
+    // x-rows measured the same.  Every per-block input is permuted here and
+    // every per-block output is permuted back (lb_download_level,
+    // lb_upload_level); per-cell results do not depend on the order.
+    std::vector<int32_t> perm((size_t)std::max(nb, 0)), inv((size_t)std::max(nb, 0));
+    for (int32_t b = 0; b < nb; b++) perm[(size_t)b] = b;
+    auto slab_key = [d](int32_t b) -> uint64_t {
+        const uint64_t x = (uint64_t)(d->map_x[b] - 1), y = (uint64_t)(d->map_y[b] - 1), z = (uint64_t)(d->map_z[b] - 1);
+        return ((z >> 2) << 48) | (y << 32) | (x << 8) | (z & 3u);
+    };
+    std::stable_sort(perm.begin(), perm.end(), [&slab_key](int32_t a, int32_t b) { return slab_key(a) < slab_key(b); });
+    for (int32_t i = 0; i < nb; i++) inv[(size_t)perm[(size_t)i]] = i;
+    std::vector<int32_t> pmx((size_t)std::max(nb, 0)), pmy(pmx.size()), pmz(pmx.size()), pnt((size_t)std::max(nb, 0) * Q);
+    std::vector<uint8_t> pobst((size_t)std::max(nb, 0) * NC);
+    std::vector<float> psponge(pobst.size()), pwall(d->wall_dist ? pobst.size() : 0);
+    std::vector<int32_t> pbp((size_t)d->ptr_dim_x * d->ptr_dim_y * d->ptr_dim_z);
+    std::vector<int32_t> pcell((size_t)std::max<int64_t>(d->n_boundary_cells, 0));
+    lb_level_desc dp = *d;
+    if (nb > 0) {
+        for (int32_t i = 0; i < nb; i++) {
+            const int32_t b = perm[(size_t)i];
+            pmx[(size_t)i] = d->map_x[b];
+            pmy[(size_t)i] = d->map_y[b];
+            pmz[(size_t)i] = d->map_z[b];
+            for (int k = 0; k < Q; k++) {
+                const int v = d->neighbor_table[(size_t)b + (size_t)nb * k];
+                pnt[(size_t)i + (size_t)nb * k] = (v >= 1 && v <= nb) ? inv[(size_t)v - 1] + 1 : v;
+            }
+            std::memcpy(&pobst[(size_t)i * NC], d->obstacle + (size_t)b * NC, NC);
+            std::memcpy(&psponge[(size_t)i * NC], d->sponge + (size_t)b * NC, NC * sizeof(float));
+            if (d->wall_dist) std::memcpy(&pwall[(size_t)i * NC], d->wall_dist + (size_t)b * NC, NC * sizeof(float));
+        }
+        for (size_t i = 0; i < pbp.size(); i++) {
+            const int v = d->block_pointer[i];
+            pbp[i] = (v >= 1 && v <= nb) ? inv[(size_t)v - 1] + 1 : v;
+        }
+        for (size_t i = 0; i < pcell.size() && d->bouzidi_cell_block; i++) {
+            const int v = d->bouzidi_cell_block[i];
+            pcell[i] = (v >= 1 && v <= nb) ? inv[(size_t)v - 1] + 1 : v;
+        }
+        dp.map_x = pmx.data();
+        dp.map_y = pmy.data();
+        dp.map_z = pmz.data();
+        dp.neighbor_table = pnt.data();
+        dp.block_pointer = pbp.empty() ? d->block_pointer : pbp.data();
+        dp.obstacle = pobst.data();
+        dp.sponge = psponge.data();
+        dp.wall_dist = d->wall_dist ? pwall.data() : nullptr;
+        if (d->bouzidi_cell_block) dp.bouzidi_cell_block = pcell.data();
+    }
+    L.perm = perm;
+    d = &dp;
     L.id = d->level_id;
     L.nb = nb;
     L.tau = d->tau;
@@ -1109,33 +1180,12 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     if (nb > 0) {
         RC(h2d(c, L.nbr, nbr.data(), nbr.size()));
         RC(h2d(c, L.borg, borg.data(), borg.size()));
-        // processing order: slabs of 4 block layers in z; inside a slab the
-        // 2-D Morton order of the (x, y) columns, the 4 blocks of a column
-        // consecutive.  x and y faces cut 128-byte lines (a line is 4 rows of
-        // 8 cells of one z-plane), so x/y neighbours should be pulled while
-        // their lines are in L2; the columns also keep z neighbours close.
-        // Measured on the 52.5M-cell cube case, finest collide: 3-D Morton
-        // 1866 us, z-layers with 2-D Morton 1850 us, these slabs 1818 us (FP32;
-        // FP16 1216 / 1207 / 1193 us).  A 2-D Hilbert curve in place of the
-        // Morton order, and x-strips 8-32 columns wide swept along y (slabs of
-        // 2, 4 or 8 layers), were all within 0.5 % of these slabs.
-        std::vector<std::pair<uint64_t, int32_t>> key((size_t)nb);
-        for (int32_t b = 0; b < nb; b++) {
-            uint64_t m = 0;
-            const uint32_t vx = (uint32_t)(d->map_x[b] - 1), vy = (uint32_t)(d->map_y[b] - 1);
-            const uint32_t vz = (uint32_t)(d->map_z[b] - 1);
-            for (int bit = 0; bit < 21; bit++)
-                m |= (uint64_t)((vx >> bit) & 1u) << (2 * bit) | (uint64_t)((vy >> bit) & 1u) << (2 * bit + 1);
-            key[(size_t)b] = {((uint64_t)(vz >> 2) << 50) | (m << 8) | (vz & 3u), b};
-        }
-        std::sort(key.begin(), key.end());
-        // interior blocks (all 26 neighbours present, ghosts included) first,
-        // then the boundary blocks, each in that order
+        // processing order: the storage order, interior blocks (all 26
+        // neighbours present, ghosts included) first, then the boundary blocks
         std::vector<int32_t> order;
         order.reserve((size_t)nb);
         for (int pass = 0; pass < 2; pass++)
-            for (int32_t i = 0; i < nb; i++) {
-                const int32_t b = key[(size_t)i].second;
+            for (int32_t b = 0; b < nb; b++) {
                 bool interior = true;
                 for (int k = 0; k < Q; k++) interior = interior && nbr[(size_t)b * Q + k] >= 0;
                 if (interior == (pass == 0)) order.push_back(b);
@@ -1371,9 +1421,10 @@ int lb_download_level(lb_ctx* c, int level_id, int field, float* dst) {
             HIPCHK(c, hipMemcpy(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost));
             for (int k = 0; k < Q; k++) {
                 const float w = wk_host(k);
-                for (int64_t b = 0; b < L.nb; b++)
-                    for (int cl = 0; cl < NC; cl++)
-                        dst[(size_t)k * n4 + (size_t)b * NC + cl] = decode_pop(tmp.data(), fidx(b, k, cl), st, w);
+                for (int64_t b = 0; b < L.nb; b++) {
+                    float* out = dst + (size_t)k * n4 + (size_t)L.perm[(size_t)b] * NC;
+                    for (int cl = 0; cl < NC; cl++) out[cl] = decode_pop(tmp.data(), fidx(b, k, cl), st, w);
+                }
             }
             return LB_OK;
         }
@@ -1381,7 +1432,10 @@ int lb_download_level(lb_ctx* c, int level_id, int field, float* dst) {
         case LB_FIELD_T: {
             const float* src = field == LB_FIELD_RHO ? L.rho[L.cur] : L.T[L.cur];
             if (!src) return fail(c, LB_EINVAL, "field not allocated");
-            HIPCHK(c, hipMemcpy(dst, src, n4 * 4, hipMemcpyDeviceToHost));
+            std::vector<float> tmp(n4);
+            HIPCHK(c, hipMemcpy(tmp.data(), src, n4 * 4, hipMemcpyDeviceToHost));
+            for (int64_t b = 0; b < L.nb; b++)
+                std::memcpy(dst + (size_t)L.perm[(size_t)b] * NC, tmp.data() + (size_t)b * NC, NC * 4);
             return LB_OK;
         }
         case LB_FIELD_VEL: {
@@ -1389,7 +1443,8 @@ int lb_download_level(lb_ctx* c, int level_id, int field, float* dst) {
             HIPCHK(c, hipMemcpy(tmp.data(), L.vel[L.cur], tmp.size() * 4, hipMemcpyDeviceToHost));
             for (int64_t b = 0; b < L.nb; b++)
                 for (int d = 0; d < 3; d++)
-                    std::memcpy(dst + (size_t)d * n4 + (size_t)b * NC, tmp.data() + ((size_t)b * 3 + d) * NC, NC * 4);
+                    std::memcpy(dst + (size_t)d * n4 + (size_t)L.perm[(size_t)b] * NC,
+                                tmp.data() + ((size_t)b * 3 + d) * NC, NC * 4);
             return LB_OK;
         }
         default:
@@ -1419,9 +1474,10 @@ int lb_upload_level(lb_ctx* c, int level_id, int field, const float* src) {
             std::vector<uint8_t> raw(n4 * Q * pop_bytes(st));
             for (int k = 0; k < Q; k++) {
                 const float w = wk_host(k);
-                for (int64_t b = 0; b < L.nb; b++)
-                    for (int cl = 0; cl < NC; cl++)
-                        encode_pop(raw.data(), fidx(b, k, cl), src[(size_t)k * n4 + (size_t)b * NC + cl], st, w);
+                for (int64_t b = 0; b < L.nb; b++) {
+                    const float* in = src + (size_t)k * n4 + (size_t)L.perm[(size_t)b] * NC;
+                    for (int cl = 0; cl < NC; cl++) encode_pop(raw.data(), fidx(b, k, cl), in[cl], st, w);
+                }
             }
             for (int i = 0; i < 2; i++) HIPCHK(c, hipMemcpy(dev[i], raw.data(), raw.size(), hipMemcpyHostToDevice));
             return LB_OK;
@@ -1430,7 +1486,9 @@ int lb_upload_level(lb_ctx* c, int level_id, int field, const float* src) {
         case LB_FIELD_T:
             pair = field == LB_FIELD_RHO ? L.rho : L.T;
             if (!pair[0]) return fail(c, LB_EINVAL, "field not allocated");
-            tmp.assign(src, src + n4);
+            tmp.resize(n4);
+            for (int64_t b = 0; b < L.nb; b++)
+                std::memcpy(tmp.data() + (size_t)b * NC, src + (size_t)L.perm[(size_t)b] * NC, NC * 4);
             n = n4;
             break;
         case LB_FIELD_VEL:
@@ -1438,7 +1496,8 @@ int lb_upload_level(lb_ctx* c, int level_id, int field, const float* src) {
             tmp.resize(n4 * 3);
             for (int64_t b = 0; b < L.nb; b++)
                 for (int d = 0; d < 3; d++)
-                    std::memcpy(tmp.data() + ((size_t)b * 3 + d) * NC, src + (size_t)d * n4 + (size_t)b * NC, NC * 4);
+                    std::memcpy(tmp.data() + ((size_t)b * 3 + d) * NC, src + (size_t)d * n4 + (size_t)L.perm[(size_t)b] * NC,
+                                NC * 4);
             n = tmp.size();
             break;
         default:

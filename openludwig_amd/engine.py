@@ -37,7 +37,7 @@ class Engine:
             raise LudwigError(f"lb_engine_create failed ({rc})")
         self.storage = storage
         self.cases: dict = {}
-        self.options = {"graph": True, "schedule": N.LB_SCHED_XCD, "waves": 5}
+        self.options = {"graph": True, "schedule": N.LB_SCHED_LINEAR, "waves": 5}
 
     def close(self) -> None:
         if self._e:
@@ -115,7 +115,7 @@ class Engine:
         bad = {arr[i] for i in range(min(n.value, cap))}
         return [s for s in self.cases if s._ctx.value in bad]
 
-    def set_options(self, graph: bool = True, schedule: int = N.LB_SCHED_XCD, waves: int = 5) -> None:
+    def set_options(self, graph: bool = True, schedule: int = N.LB_SCHED_LINEAR, waves: int = 5) -> None:
         self._check(self.lib.lb_engine_set_options(self._e, int(graph), int(schedule), int(waves)),
                     "lb_engine_set_options")
         self.options = {"graph": bool(graph), "schedule": int(schedule), "waves": int(waves)}
@@ -129,7 +129,7 @@ class Engine:
         import time
         if candidates is None:
             candidates = [dict(graph=g, schedule=sc, waves=w) for g, sc, w in
-                          itertools.product((True, False), (N.LB_SCHED_XCD, N.LB_SCHED_LINEAR), (5, 6))]
+                          itertools.product((True, False), (N.LB_SCHED_LINEAR, N.LB_SCHED_XCD), (5, 6))]
         timings = []
         for opt in candidates:
             if not any(not self.done(s) for s in self.cases):
