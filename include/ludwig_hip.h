@@ -308,9 +308,13 @@ int lb_engine_check_finite(lb_engine* e, lb_ctx** diverged, int32_t cap, int32_t
  * graph (1: replay a captured hipGraph per step), the work-group schedule
  * of the stream-collide launches (LB_SCHED_*; default LB_SCHED_LINEAR, the
  * only one lb_step_batch uses) and the occupancy of the interior-block
- * stream-collide kernel, 5 or 6 waves per SIMD.                          */
-#define LB_SCHED_XCD 0    /* work-groups dealt to XCDs in contiguous runs */
-#define LB_SCHED_LINEAR 1 /* work-group i takes half-tile i               */
+ * stream-collide kernel, 5 or 6 waves per SIMD (the persistent form runs
+ * at 5).                                                                 */
+#define LB_SCHED_XCD 0        /* work-groups dealt to XCDs in contiguous runs  */
+#define LB_SCHED_LINEAR 1     /* work-group i takes half-tile i                */
+#define LB_SCHED_PERSISTENT 2 /* 5 work-groups per CU, each looping over the
+                               * half-tiles in order (stream-collide only; the
+                               * thermal kernel runs LB_SCHED_LINEAR)          */
 int lb_engine_set_options(lb_engine* e, int32_t use_graph, int32_t schedule, int32_t waves);
 
 /* ------------------------------------------------------------------------

@@ -116,6 +116,7 @@ HIP_EXPORTS = {
 
 LB_SCHED_XCD = 0
 LB_SCHED_LINEAR = 1
+LB_SCHED_PERSISTENT = 2
 
 SETUP_EXPORTS = {
     "lbs_last_error": (C.c_char_p, []),

@@ -66,8 +66,12 @@ def build_hip(force: bool = True) -> Path:
     objs = []
     for ns, contract in (("lbk_strict", "off"), ("lbk_fast", "fast")):
         obj = OUT / f"lb_kernels_{ns}.o"
-        _run([HIPCC, *DEVICE_FLAGS, f"-ffp-contract={contract}", f"-DLBK_NS={ns}", "-c",
-              str(CSRC / "lb_kernels.hip"), "-o", str(obj)])
+        # No machine-level loop-invariant hoisting: in the persistent
+        # stream-collide kernel it lifts the tile's constants out of the tile
+        # loop and keeps them live across it (VGPR spills 63 -> 153 dwords).
+        # The one-tile kernels have no such loop and compile the same.
+        _run([HIPCC, *DEVICE_FLAGS, "-mllvm", "-disable-machine-licm", f"-ffp-contract={contract}",
+              f"-DLBK_NS={ns}", "-c", str(CSRC / "lb_kernels.hip"), "-o", str(obj)])
         objs.append(obj)
     for src in ("lb_api", "lb_probe", "lb_setup_gpu"):
         obj = OUT / f"{src}.o"

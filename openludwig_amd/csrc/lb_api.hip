@@ -477,7 +477,8 @@ int step_level(lb_ctx* c, int li, int64_t j) {
         if (fill) HIPCHK(c, c->k->ghost_fill(las, st, L.wgh, L.n_gchunks, j, storage, 0, c->stream));
         hipEvent_t e1 = nullptr;
         RC(profile_begin(c, L, 0, &e1));
-        HIPCHK(c, c->k->collide(las, st, L.wcol, L.n_int, j, storage, L.wide ? 1 : 0, 0, LB_SCHED_LINEAR, 5, c->stream));
+        HIPCHK(c, c->k->collide(las, st, L.wcol, L.n_int, j, storage, L.wide ? 1 : 0, 0, LB_SCHED_LINEAR, 5,
+                                c->stream));
         HIPCHK(c, c->k->collide(las, st, L.wcol + L.n_int, L.nb - L.n_int, j, storage, L.wide ? 1 : 0, 1,
                                 LB_SCHED_LINEAR, 5, c->stream));
         if (e1) HIPCHK(c, hipEventRecord(e1, c->stream));
@@ -781,15 +782,19 @@ hipError_t engine_op(lb_engine* e, int d, int64_t j) {
     // one ghost pass per parent step fills the inputs of both child substeps
     const bool fill = (j & 1) == 0;
     if (W.ngh && fill) ETRY(k->ghost_fill(e->las, st, W.gh, W.ngh, j, e->storage, 0, e->stream));
-    if (W.ncol) ETRY(k->collide(e->las, st, W.col, W.ncol, j, e->storage, W.wide, 0, e->sched, e->waves, e->stream));
+    if (W.ncol)
+        ETRY(k->collide(e->las, st, W.col, W.ncol, j, e->storage, W.wide, 0, e->sched, e->waves, e->stream));
     if (W.ncolb)
-        ETRY(k->collide(e->las, st, W.colb, W.ncolb, j, e->storage, W.wide, 1, e->sched, e->waves, e->stream));
+        ETRY(k->collide(e->las, st, W.colb, W.ncolb, j, e->storage, W.wide, 1, e->sched, e->waves,
+                        e->stream));
     if (W.nlk) ETRY(k->bouzidi(e->las, st, W.lk, W.nlk, j, e->storage, 0, e->stream));
     if (W.ntcol || W.ntcolb) {
+        // the thermal kernel has no persistent form
+        const int tsched = e->sched == LB_SCHED_PERSISTENT ? LB_SCHED_LINEAR : e->sched;
         if (W.ntgh && fill) ETRY(k->ghost_fill(e->las, st, W.tgh, W.ntgh, j, e->storage, 1, e->stream));
-        if (W.ntcol) ETRY(k->thermal(e->las, st, W.tcol, W.ntcol, j, e->storage, W.wide_g, 0, e->sched, e->stream));
+        if (W.ntcol) ETRY(k->thermal(e->las, st, W.tcol, W.ntcol, j, e->storage, W.wide_g, 0, tsched, e->stream));
         if (W.ntcolb)
-            ETRY(k->thermal(e->las, st, W.tcolb, W.ntcolb, j, e->storage, W.wide_g, 1, e->sched, e->stream));
+            ETRY(k->thermal(e->las, st, W.tcolb, W.ntcolb, j, e->storage, W.wide_g, 1, tsched, e->stream));
         if (W.ntlk) ETRY(k->bouzidi(e->las, st, W.tlk, W.ntlk, j, e->storage, 1, e->stream));
     }
     if (d < e->lmax) {
@@ -1838,7 +1843,9 @@ int lb_engine_check_finite(lb_engine* e, lb_ctx** diverged, int32_t cap, int32_t
 }
 
 int lb_engine_set_options(lb_engine* e, int32_t use_graph, int32_t schedule, int32_t waves) {
-    if (!e || (schedule != LB_SCHED_XCD && schedule != LB_SCHED_LINEAR) || waves < 5 || waves > 6) return LB_EINVAL;
+    if (!e || (schedule != LB_SCHED_XCD && schedule != LB_SCHED_LINEAR && schedule != LB_SCHED_PERSISTENT) ||
+        waves < 5 || waves > 6)
+        return LB_EINVAL;
     EHIPCHK(e, hipSetDevice(e->device));
     EHIPCHK(e, hipStreamSynchronize(e->stream));
     e->use_graph = use_graph != 0;

@@ -21,6 +21,7 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "ludwig_hip.h"
 #include "lb_kernels.h"
 
 #ifndef LBK_NS
@@ -483,8 +484,8 @@ __device__ __forceinline__ bool schedule(const Work* __restrict__ work, int n_ti
 // in [0, buffer size), which the host guarantees fits OffT.
 template <class OffT>
 __device__ __forceinline__ void build_pull_table(const int32_t* __restrict__ row, int b, int half, OffT esz,
-                                                 OffT* ptab, uint64_t* smask) {
-    const int t = threadIdx.x;
+                                                 OffT* ptab, uint64_t* smask, int tid) {
+    const int t = tid;
     bool missing = false;
     if (t < 108) {
         const int w = t / 27, e = t - 27 * w;
@@ -525,8 +526,8 @@ __device__ __forceinline__ uint32_t wave_miss(const uint64_t* smask, int w) {
 // ghost block, which holds no velocity) use the cell's own velocity.
 template <class OffT>
 __device__ __forceinline__ void build_vel_table(const int32_t* __restrict__ row, int nb, int b, int half,
-                                                OffT* vtab) {
-    const int t = threadIdx.x - 128;
+                                                OffT* vtab, int tid) {
+    const int t = tid - 128;
     if (t >= 0 && t < 32) {
         const int w = t >> 3, n = t & 7;
         const OffT own = (OffT)b * (OffT)(3 * NC * 4);
@@ -572,8 +573,8 @@ struct VelTile {
     int hz, hy, hx;
 };
 template <class OffT>
-__device__ __forceinline__ VelTile vel_tile_load(const float* vel, const OffT* vtab, int half, int c) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+__device__ __forceinline__ VelTile vel_tile_load(const float* vel, const OffT* vtab, int half, int c, int tid) {
+    const int lane = tid & 63, wv = tid >> 6;
     VelTile t;
     const GF* p = gaddr(vel, vtab[wv * 8 + 6] + (OffT)(4 * c));
     t.o[0] = p[0], t.o[1] = p[NC], t.o[2] = p[2 * NC];
@@ -600,8 +601,8 @@ __device__ __forceinline__ VelTile vel_tile_load(const float* vel, const OffT* v
 }
 // Second half: the values into the tile, once the caller has issued its
 // population loads, so that the wait and the barrier overlap them.
-__device__ __forceinline__ void vel_tile_store(const VelTile& t, int c, float* vt) {
-    const int wv = threadIdx.x >> 6, x = c & 7, y = (c >> 3) & 7;
+__device__ __forceinline__ void vel_tile_store(const VelTile& t, int c, float* vt, int tid) {
+    const int wv = tid >> 6, x = c & 7, y = (c >> 3) & 7;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         vt[vt_index(d, wv, y, x)] = t.o[d];
@@ -623,12 +624,14 @@ __device__ __forceinline__ void vel_tile_store(const VelTile& t, int c, float* v
 // operations (tested bit-identical); a 4-wave build contracts differently and
 // is not offered.  Measured on the 52.5M-cell cube case with the staged
 // stores: 5 waves is best (6 waves spills more: FP32 +5 %, FP16 +7 %).
-template <int ST, class OffT, bool BND, int W>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
-collide_kernel(const LevelArgs* __restrict__ las,
-                                                      const CaseState* __restrict__ states,
-                                                      const Work* __restrict__ work, int32_t n_tiles, int64_t j,
-                                                      int32_t sched) {
+//
+// collide_tile is one half-block tile (work-list entry wk, half `half`);
+// collide_kernel runs one tile per work-group, collide_kernel_persistent
+// (LB_SCHED_PERSISTENT) runs a grid of as many work-groups as fit the
+// device at once, each looping over the tiles in order.
+template <int ST, class OffT, bool BND>
+__device__ __forceinline__ void collide_tile(const LevelArgs* __restrict__ las, const CaseState* __restrict__ states,
+                                             const Work& wk, int half, int64_t j, int tid) {
     __shared__ OffT ptab[108];
     __shared__ OffT vtab[32];
     __shared__ uint64_t smask[2];
@@ -647,22 +650,19 @@ collide_kernel(const LevelArgs* __restrict__ las,
     constexpr int ROWS = ST == 0 ? 14 : 27;
     __shared__ __attribute__((aligned(16))) Stored<ST> sst[STAGE ? 4 * ROWS * 64 : 8];
     __shared__ float vt[VTILE ? 3 * VT_SIZE : 1];
-    Work wk;
-    int half;
-    if (!schedule(work, n_tiles, sched, wk, half)) return;
     const LevelArgs& L = las[wk.li];
     const int b = wk.item;
     constexpr int ESZ = ST == 0 ? 4 : 2;
-    const int c = (half << 8) | threadIdx.x;
+    const int c = (half << 8) | tid;
     // the neighbour-row loads go out together with the case record's
-    build_pull_table<OffT>(wk.nbr_row, b, half, (OffT)ESZ, ptab, smask);
-    build_vel_table<OffT>(wk.nbr_row, L.nb, b, half, vtab);
+    build_pull_table<OffT>(wk.nbr_row, b, half, (OffT)ESZ, ptab, smask, tid);
+    build_vel_table<OffT>(wk.nbr_row, L.nb, b, half, vtab, tid);
     const CaseState S = case_state(states + wk.ci);
     __syncthreads();
     if (!live(L, S, j)) return;
     const CollideArgs a = collide_view(L, S, j);
     const int x = c & 7, y = (c >> 3) & 7, z = c >> 6;
-    const int wv = threadIdx.x >> 6;
+    const int wv = tid >> 6;
     const int gx = a.borg[b * 3 + 0] + x + 1;
     const int gy = a.borg[b * 3 + 1] + y + 1;
     const int gz = a.borg[b * 3 + 2] + z + 1;
@@ -676,7 +676,7 @@ collide_kernel(const LevelArgs* __restrict__ las,
     const int oyp = y == 0 ? -1 : 0, oym = y == 7 ? 1 : 0;
     const OffT cb = (OffT)c * (OffT)ESZ;
     VelTile vtl;
-    if constexpr (VTILE) vtl = vel_tile_load<OffT>(a.vel_in, vtab, half, c);
+    if constexpr (VTILE) vtl = vel_tile_load<OffT>(a.vel_in, vtab, half, c, tid);
     float A[27];
 #pragma unroll
     for (int k = 0; k < 27; k++) {
@@ -690,7 +690,7 @@ collide_kernel(const LevelArgs* __restrict__ las,
     // WALE stencil of the previous velocity, loaded together with the populations.
     float VN[6][3];
     if constexpr (VTILE) {
-        vel_tile_store(vtl, c, vt);
+        vel_tile_store(vtl, c, vt, tid);
         __syncthreads();
 #pragma unroll
         for (int n = 0; n < 6; n++) {
@@ -750,11 +750,11 @@ collide_kernel(const LevelArgs* __restrict__ las,
 #pragma unroll 1
             for (int k = 0; k < 27; k++)
                 if (miss & (1u << k))
-                    sfix[k * 256 + threadIdx.x] =
+                    sfix[k * 256 + tid] =
                         bc_value<ST>(a, k, b, c, gx - CX(k), gy - CY(k), gz - CZ(k), gy, gz);
 #pragma unroll
             for (int k = 0; k < 27; k++)
-                if (miss & (1u << k)) A[k] = sfix[k * 256 + threadIdx.x];
+                if (miss & (1u << k)) A[k] = sfix[k * 256 + tid];
         }
     }
     float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
@@ -1069,12 +1069,69 @@ collide_kernel(const LevelArgs* __restrict__ las,
     if constexpr (STAGE) stream_tile<ST, ROWS>(fo, b, c, A, sst + wv * (ROWS * 64));
 }
 
+template <int ST, class OffT, bool BND, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
+collide_kernel(const LevelArgs* __restrict__ las, const CaseState* __restrict__ states,
+               const Work* __restrict__ work, int32_t n_tiles, int64_t j, int32_t sched) {
+    Work wk;
+    int half;
+    if (!schedule(work, n_tiles, sched, wk, half)) return;
+    collide_tile<ST, OffT, BND>(las, states, wk, half, j, (int)threadIdx.x);
+}
+
+// Persistent form: work-group g takes tiles g, g + G, g + 2G, ... in order.
+// All waves of a work-group make the same passes, and a barrier ends each
+// pass, so the LDS tables of a tile are not rewritten while any wave still
+// reads them.  A tile's early exits are uniform over the work-group (a case
+// past its last step) or come after the tile's last barrier (a solid lane's
+// bounce-back in the boundary form), so every wave reaches every barrier.
+template <int ST, class OffT, bool BND>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5)))
+collide_kernel_persistent(const LevelArgs* __restrict__ las, const CaseState* __restrict__ states,
+                          const Work* __restrict__ work, int32_t n_tiles, int64_t j) {
+    for (unsigned logical = blockIdx.x; logical < 2u * (unsigned)n_tiles; logical += gridDim.x) {
+        const Work wk = work[logical >> 1];
+        // The work-item index is made opaque on every pass, so that nothing
+        // derived from it (lane geometry, table addresses) is hoisted out of
+        // the loop and kept live across the whole tile.
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        collide_tile<ST, OffT, BND>(las, states, wk, (int)(logical & 1u), j, tid);
+        __syncthreads();
+    }
+}
+
 inline unsigned padded_groups(int nb) { return ((unsigned)nb * 2u + 7u) & ~7u; }
+
+// Work-groups of a persistent launch: 5 per CU (one wave per SIMD each, at
+// 5 waves per SIMD), no more than there are tiles.
+inline unsigned persistent_groups(int n_tiles) {
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cus[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    const unsigned full = 5u * (unsigned)cus[dev];
+    const unsigned need = padded_groups(n_tiles);
+    return need < full ? need : full;
+}
 
 template <int ST, class OffT>
 void collide_launch(const LevelArgs* las, const CaseState* states, const Work* work, int32_t n_tiles, int64_t j,
                     int boundary, int sched, int waves, hipStream_t s) {
-    const dim3 g(padded_groups(n_tiles)), t(256);
+    const dim3 t(256);
+    if (sched == LB_SCHED_PERSISTENT) {
+        const dim3 g(persistent_groups(n_tiles));
+        if (boundary)
+            hipLaunchKernelGGL((collide_kernel_persistent<ST, OffT, true>), g, t, 0, s, las, states, work, n_tiles, j);
+        else
+            hipLaunchKernelGGL((collide_kernel_persistent<ST, OffT, false>), g, t, 0, s, las, states, work, n_tiles, j);
+        return;
+    }
+    const dim3 g(padded_groups(n_tiles));
     if (boundary)
         hipLaunchKernelGGL((collide_kernel<ST, OffT, true, 5>), g, t, 0, s, las, states, work, n_tiles, j, sched);
     else if (waves == 6)
@@ -1132,7 +1189,7 @@ __global__ void __launch_bounds__(256) thermal_kernel(const LevelArgs* __restric
     const int b = wk.item;
     const int c = (half << 8) | threadIdx.x;
     constexpr int ESZ = ST == 0 ? 4 : 2;
-    build_pull_table<OffT>(wk.nbr_row, b, half, (OffT)ESZ, ptab, smask);
+    build_pull_table<OffT>(wk.nbr_row, b, half, (OffT)ESZ, ptab, smask, (int)threadIdx.x);
     const CaseState S = case_state(states + wk.ci);
     __syncthreads();
     if (!live(L, S, j)) return;

@@ -130,6 +130,7 @@ class Engine:
         if candidates is None:
             candidates = [dict(graph=g, schedule=sc, waves=w) for g, sc, w in
                           itertools.product((True, False), (N.LB_SCHED_LINEAR, N.LB_SCHED_XCD), (5, 6))]
+            candidates += [dict(graph=g, schedule=N.LB_SCHED_PERSISTENT, waves=5) for g in (True, False)]
         timings = []
         for opt in candidates:
             if not any(not self.done(s) for s in self.cases):

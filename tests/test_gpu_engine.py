@@ -1,8 +1,8 @@
 """Batched engine (lb_engine_*): a case stepped through the engine equals the
 same case stepped by lb_step_batch, bit for bit, and a case's results are
 the same alone and in a mix of cases with other level counts, sizes and step
-counts (with and without the captured graph, under both work-group
-schedules)."""
+counts (with and without the captured graph, under each work-group
+schedule, the persistent stream-collide kernel included)."""
 import numpy as np
 import pytest
 
@@ -30,8 +30,11 @@ def _assert_same(a, b, what):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("storage", [N.LB_STORAGE_FP32, N.LB_STORAGE_FP16])
-def test_engine_matches_step_batch(hip_device, storage):
+@pytest.mark.parametrize("storage,sched", [(N.LB_STORAGE_FP32, N.LB_SCHED_LINEAR),
+                                           (N.LB_STORAGE_FP16, N.LB_SCHED_LINEAR),
+                                           (N.LB_STORAGE_FP32, N.LB_SCHED_PERSISTENT),
+                                           (N.LB_STORAGE_FP16, N.LB_SCHED_PERSISTENT)])
+def test_engine_matches_step_batch(hip_device, storage, sched):
     cfg, grids, params, mesh = _case(8, 2)
     n = 20  # two full batches of 8 and a partial one
     ref = Solver(cfg, grids, params, mesh, storage=storage)
@@ -42,6 +45,7 @@ def test_engine_matches_step_batch(hip_device, storage):
     s = Solver(cfg, grids, params, mesh, storage=storage)
     s.init_equilibrium()
     e = Engine(storage=storage)
+    e.set_options(graph=True, schedule=sched, waves=5)
     e.add(s, n)
     e.run_to_completion()
     assert e.progress(s) == n
@@ -51,7 +55,8 @@ def test_engine_matches_step_batch(hip_device, storage):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("graph,sched,waves", [(True, N.LB_SCHED_XCD, 5), (False, N.LB_SCHED_LINEAR, 6)])
+@pytest.mark.parametrize("graph,sched,waves", [(True, N.LB_SCHED_XCD, 5), (False, N.LB_SCHED_LINEAR, 6),
+                                               (True, N.LB_SCHED_PERSISTENT, 5)])
 def test_engine_alone_vs_mixed(hip_device, graph, sched, waves):
     specs = [((8, 2), 24), ((16, 3), 12), ((8, 3), 16)]
     built = [(_case(*rl), n) for rl, n in specs]
