@@ -67,7 +67,10 @@ def ramp_velocity(u_target: float, batch_end: int, ramp_steps: int) -> float:
     return float(np.float32(u_target))
 
 
-def level_desc(lev: BlockLevel, keep: list) -> N.LbLevelDesc:
+def level_desc(lev: BlockLevel, keep: list, dense_q: bool = False) -> N.LbLevelDesc:
+    """The lb_level_desc of a level.  dense_q passes the Bouzidi q map as the
+    reference holds it (level.bouzidi_q_map, (8,8,8,nb,27) Float16 bits, 0
+    where no link) instead of the per-cell list."""
     d = N.LbLevelDesc()
     nb = lev.n_blocks
     mx = np.ascontiguousarray(lev.coords[:, 0]); my = np.ascontiguousarray(lev.coords[:, 1])
@@ -99,7 +102,15 @@ def level_desc(lev: BlockLevel, keep: list) -> N.LbLevelDesc:
         d.bouzidi_cell_block = N.ptr(cb, C.c_int32)
         d.bouzidi_cell_x, d.bouzidi_cell_y, d.bouzidi_cell_z = (N.ptr(cx, C.c_int8), N.ptr(cy, C.c_int8),
                                                                 N.ptr(cz, C.c_int8))
-        d.bouzidi_q_cells = N.ptr(q, C.c_uint16)
+        if dense_q:
+            dense = np.zeros((27, nb, 512), dtype=np.uint16)
+            cells = lev.bouzidi_cells.astype(np.int64)
+            cl = (cells[:, 1] - 1) + 8 * (cells[:, 2] - 1) + 64 * (cells[:, 3] - 1)
+            dense[:, cells[:, 0] - 1, cl] = q.T
+            keep.append(dense)
+            d.bouzidi_q_dense = N.ptr(dense, C.c_uint16)
+        else:
+            d.bouzidi_q_cells = N.ptr(q, C.c_uint16)
     return d
 
 
@@ -161,7 +172,7 @@ def footprint(cfg: CaseConfig, grids: list, params: DomainParameters, mesh: Solv
 class Solver:
     def __init__(self, cfg: CaseConfig, grids: list, params: DomainParameters, mesh: SolverMesh | None = None,
                  device: int = 0, strict_fp: bool = False, storage: int = N.LB_STORAGE_FP32,
-                 wide_offsets: bool = False):
+                 wide_offsets: bool = False, dense_q: bool = False):
         self.cfg = cfg
         self.grids = grids
         self.params = params
@@ -177,7 +188,7 @@ class Solver:
         self.t_next = 1
         keep: list = []
         for lev in grids:
-            d = level_desc(lev, keep)
+            d = level_desc(lev, keep, dense_q)
             self._check(self.lib.lb_level_upload(self._ctx, C.byref(d)), "lb_level_upload")
         if mesh is not None and cfg.forces_enabled:
             arr = [np.ascontiguousarray(a, dtype=np.float32) for a in

@@ -165,3 +165,16 @@ def test_engine_mix_two_to_five_levels_vs_oracle(hip_device):
         _assert_bitexact(compare(s, o, grids), f"engine {len(grids)}L")
         o.close()
         s.close()
+
+
+@pytest.mark.gpu
+def test_dense_bouzidi_q_map_strict_bitexact(hip_device):
+    """The Bouzidi q map passed dense, in the form the reference holds it
+    (level.bouzidi_q_map, (8,8,8,nb,27) in the caller's block order, read
+    while the blocks are stored in the device's order), gives the oracle's
+    results bit for bit, as the per-cell list does."""
+    cfg, mesh = cases.tiny_sphere(8, 2)
+    grids, params, mesh = build_case(cfg, mesh)
+    assert grids[-1].n_boundary_cells > 0
+    s, o = run_pair(cfg, grids, params, mesh, steps=12, strict=True, dense_q=True)
+    _assert_bitexact(compare(s, o, grids), "dense q map")
