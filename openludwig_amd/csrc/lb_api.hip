@@ -902,8 +902,8 @@ __global__ void stats_reduce_kernel(const double* __restrict__ part, int rows, d
 }  // namespace
 
 namespace lbsync {
-std::shared_mutex& capture_mutex() {
-    static std::shared_mutex m;
+CaptureLock& capture_mutex() {
+    static CaptureLock m;
     return m;
 }
 }  // namespace lbsync
@@ -1713,7 +1713,7 @@ int lb_engine_step(lb_engine* e, int32_t n) {
     if (e->use_graph && !e->exec) {
         // GPU calls of other threads wait for the capture (lb_sync.h);
         // relaxed mode in addition lets them run host-side work meanwhile
-        std::unique_lock<std::shared_mutex> capture(lbsync::capture_mutex());
+        std::unique_lock<lbsync::CaptureLock> capture(lbsync::capture_mutex());
         EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
         const hipError_t r = engine_issue_step(e);
         hipGraph_t g = nullptr;
