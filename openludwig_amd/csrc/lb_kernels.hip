@@ -18,6 +18,7 @@
 // lbk_fast).
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <cstdint>
 #include <type_traits>
 
@@ -1106,15 +1107,15 @@ inline unsigned padded_groups(int nb) { return ((unsigned)nb * 2u + 7u) & ~7u; }
 // Work-groups of a persistent launch: 5 per CU (one wave per SIMD each, at
 // 5 waves per SIMD), no more than there are tiles.
 inline unsigned persistent_groups(int n_tiles) {
-    static int cus[64] = {0};
+    static std::atomic<int> cus[64] = {};  // CU count per device, queried once
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if (cus[dev] == 0) {
-        int n = 0;
+    int n = cus[dev].load(std::memory_order_relaxed);
+    if (n == 0) {
         if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-        cus[dev] = n;
+        cus[dev].store(n, std::memory_order_relaxed);
     }
-    const unsigned full = 5u * (unsigned)cus[dev];
+    const unsigned full = 5u * (unsigned)n;
     const unsigned need = padded_groups(n_tiles);
     return need < full ? need : full;
 }
