@@ -9,6 +9,10 @@ starts from rest as the reference's init_eq! does.
 
 --workload naca-thermal measures the compressible thermal form instead: an
 extruded NACA 0012 wing at alpha 2 deg with the thermal DDF, 29.4 M cells.
+--workload sphere is the small CPU-runnable case: a 1 m sphere at Re 1e3,
+coarse 64x32x32, 3 levels, 1.34 M cells, 2,000 coarse steps, no wall model
+(249 B per cell update in FP32); with --cpu-steps -1 the C oracle runs the
+whole case on the host as well, force samples included.
 
 A "step" is one coarse time step: the full multilevel recursion (15
 substeps over 4 levels), issued in batches of 8 like execute_timestep_batch!.
@@ -28,7 +32,8 @@ f - w; on the finest level the 4 B density store happens on one substep of
 the 8 per coarse step, so its launches average 249.5 and 141.5 B) / its
 average launch time from HIP events on the library's stream.
 cpu_baseline = the C oracle (the restated reference algorithm) on the host
-cores over one coarse step of the same workload.  serving = the serving
+cores over --cpu-steps coarse steps of the same workload (default one; -1:
+the whole run with the force samples, whose averaged Cd/Cl it reports).  serving = the serving
 comparison of openludwig_amd.serving.compare (FP16 continuous batching vs
 plain FP32 sequential) on the serving request set: 64 requests of 1-40 M
 cells, 2-5 levels.
@@ -81,13 +86,17 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=8)
-    ap.add_argument("--workload", default="cube", choices=["cube", "naca-thermal"],
-                    help="cube: plain drop-in case (52.5M cells); naca-thermal: compressible thermal wing (29.4M cells)")
-    ap.add_argument("--resolution", type=int, default=0, help="0: the workload's own (cube 160, wing 288)")
+    ap.add_argument("--workload", default="cube", choices=["cube", "sphere", "naca-thermal"],
+                    help="cube: plain drop-in case (52.5M cells); sphere: the small CPU-runnable case "
+                         "(Re 1e3, 1.34M cells); naca-thermal: compressible thermal wing (29.4M cells)")
+    ap.add_argument("--resolution", type=int, default=0,
+                    help="0: the workload's own (cube 160, sphere 16, wing 288)")
     ap.add_argument("--strict", action="store_true", help="use the no-FMA build")
     ap.add_argument("--storage", default="fp32", choices=sorted(B_ALG_WMLES),
                     help="population storage (fp16/bf16: offsets f - w, FP32 arithmetic)")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="coarse steps timed for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="coarse steps timed for the CPU baseline (0: skip; -1: the whole run, with the force "
+                         "samples)")
     ap.add_argument("--probe-gib", type=float, default=2.0,
                     help="buffer size of the HBM bandwidth probe (0: skip)")
     ap.add_argument("--serving-requests", type=int, default=64,
@@ -115,6 +124,10 @@ def main() -> int:
         args.resolution = args.resolution or 160
         cfg, mesh = cases.cube_re1e5(args.resolution)
         wname = "cube Re 1e5, 49,152-tri cube, cumulant+WALE+WMLES, Bouzidi finest level"
+    elif args.workload == "sphere":
+        args.resolution = args.resolution or 16
+        cfg, mesh = cases.sphere_re1e3(args.resolution)
+        wname = "sphere D 1 m Re 1e3, 5,120-tri icosphere, cumulant+WALE, Bouzidi finest level"
     else:
         args.resolution = args.resolution or 288
         cfg, mesh = cases.naca_wing_thermal(args.resolution)
@@ -130,7 +143,8 @@ def main() -> int:
     spinup = total - args.warmup - args.steps
     cfg = cfg.replace(ramp_steps=args.ramp, steps=total)
     solver = Solver(cfg, grids, params, mesh, strict_fp=args.strict, storage=STORAGE_CODE[args.storage])
-    b_alg = B_ALG_WMLES[args.storage]
+    # without the wall model the kernel reads no wall distance (4 B less)
+    b_alg = B_ALG_WMLES[args.storage] - (0 if cfg.wall_model_enabled else 4)
     solver.init_equilibrium()
     mass0 = solver.mass()
     batch = cfg.gpu_async_depth
@@ -241,24 +255,43 @@ def main() -> int:
                 f"copy NT {pb[3]:.0f}, copy NT grid-stride {pb[4]:.0f} GB/s")
 
     cpu = None
-    if args.cpu_steps > 0:
+    n_cpu = t_end if args.cpu_steps < 0 else args.cpu_steps
+    if n_cpu > 0:
         sys.path.insert(0, str(ROOT / "oracle"))
-        from oracle import Oracle
+        from oracle import Oracle, oracle_forces
         del solver
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         o = Oracle(cfg, grids, params, "f32")
         o.init_equilibrium()
-        tc = time.perf_counter()
-        o.step_batch(1, args.cpu_steps, ramp_velocity(cfg.u_lattice, args.cpu_steps, cfg.ramp_steps))
-        tcpu = time.perf_counter() - tc
+        # the GPU run's cadence: batches of gpu_async_depth at the inlet
+        # velocity of each batch end, force samples after the ramp (their
+        # time is left out of the rate, as the GPU's timed window holds none)
+        cacc = ForceAccumulator()
+        tcpu = 0.0
+        t = 1
+        while t <= n_cpu:
+            be = min(t + batch - 1, n_cpu)
+            tc = time.perf_counter()
+            o.step_batch(t, be - t + 1, ramp_velocity(cfg.u_lattice, be, cfg.ramp_steps))
+            tcpu += time.perf_counter() - tc
+            d = (be // args.diag) * args.diag
+            if t <= d <= be and d > cfg.ramp_steps:
+                cacc.add(oracle_forces(o, cfg, params, mesh, grids[-1].dx))
+            if n_cpu > batch and t <= d <= be:
+                log(f"cpu baseline: coarse step {be} of {n_cpu}, {tcpu:.1f}s")
+            t = be + 1
         o.close()
-        c_true, _ = mlups(grids, args.cpu_steps, tcpu)
+        c_true, _ = mlups(grids, n_cpu, tcpu)
         cpu = {"value": round(c_true, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} coarse step(s) of the same {sum(n_cells) / 1e6:.1f}M-cell, "
+               "sample": f"{n_cpu} coarse step(s) of the same {sum(n_cells) / 1e6:.1f}M-cell, "
                          f"{len(grids)}-level workload "
-                         f"({sum(c * 2 ** i for i, c in enumerate(n_cells)) * args.cpu_steps / 1e6:.0f}M cell "
+                         f"({sum(c * 2 ** i for i, c in enumerate(n_cells)) * n_cpu / 1e6:.0f}M cell "
                          f"updates), C oracle (FP32 mirror, OpenMP)", "seconds": round(tcpu, 2),
                "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+        if cacc.n_samples:
+            cc = averaged_coefficients(params, cacc.sums, cacc.n_samples)
+            cpu.update({"Cd_avg": float(f"{cc['Cd_avg']:.6g}"), "Cl_avg": float(f"{cc['Cl_avg']:.6g}"),
+                        "force_samples": cacc.n_samples})
         log(f"cpu baseline {c_true:.1f} MLUPS on {threads} threads ({tcpu:.1f}s)")
 
     serving = None
