@@ -81,6 +81,52 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_baseline(cfg, grids, params, mesh, n_steps: int, diag: int) -> dict:
+    """The CPU baseline: the C oracle (FP32 mirror of the restated reference
+    algorithm, OpenMP over blocks) on the host cores over coarse steps
+    1..n_steps of the case, with the GPU run's cadence -- batches of
+    gpu_async_depth at the inlet velocity of each batch end, force samples
+    every `diag` coarse steps after the ramp, whose time is left out of the
+    rate as the GPU's timed window holds none.  Returns the cpu_baseline
+    object of the JSON line (with the averaged Cd/Cl when samples were taken)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from oracle import Oracle, oracle_forces
+
+    from openludwig_amd.solver import ForceAccumulator, averaged_coefficients, mlups, ramp_velocity
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    batch = cfg.gpu_async_depth
+    n_cells = [g.n_cells for g in grids]
+    o = Oracle(cfg, grids, params, "f32")
+    o.init_equilibrium()
+    acc = ForceAccumulator()
+    tcpu = 0.0
+    t = 1
+    while t <= n_steps:
+        be = min(t + batch - 1, n_steps)
+        tc = time.perf_counter()
+        o.step_batch(t, be - t + 1, ramp_velocity(cfg.u_lattice, be, cfg.ramp_steps))
+        tcpu += time.perf_counter() - tc
+        d = (be // diag) * diag
+        if t <= d <= be and d > cfg.ramp_steps:
+            acc.add(oracle_forces(o, cfg, params, mesh, grids[-1].dx))
+        if n_steps > batch and t <= d <= be:
+            log(f"cpu baseline: coarse step {be} of {n_steps}, {tcpu:.1f}s")
+        t = be + 1
+    o.close()
+    c_true, _ = mlups(grids, n_steps, tcpu)
+    cpu = {"value": round(c_true, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+           "sample": f"{n_steps} coarse step(s) of the same {sum(n_cells) / 1e6:.1f}M-cell, {len(grids)}-level "
+                     f"workload ({sum(c * 2 ** i for i, c in enumerate(n_cells)) * n_steps / 1e6:.0f}M cell "
+                     f"updates), C oracle (FP32 mirror, OpenMP)", "seconds": round(tcpu, 2),
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+    if acc.n_samples:
+        cc = averaged_coefficients(params, acc.sums, acc.n_samples)
+        cpu.update({"Cd_avg": float(f"{cc['Cd_avg']:.6g}"), "Cl_avg": float(f"{cc['Cl_avg']:.6g}"),
+                    "force_samples": acc.n_samples})
+    log(f"cpu baseline {c_true:.1f} MLUPS on {threads} threads ({tcpu:.1f}s)")
+    return cpu
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,42 +303,8 @@ def main() -> int:
     cpu = None
     n_cpu = t_end if args.cpu_steps < 0 else args.cpu_steps
     if n_cpu > 0:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        from oracle import Oracle, oracle_forces
         del solver
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        o = Oracle(cfg, grids, params, "f32")
-        o.init_equilibrium()
-        # the GPU run's cadence: batches of gpu_async_depth at the inlet
-        # velocity of each batch end, force samples after the ramp (their
-        # time is left out of the rate, as the GPU's timed window holds none)
-        cacc = ForceAccumulator()
-        tcpu = 0.0
-        t = 1
-        while t <= n_cpu:
-            be = min(t + batch - 1, n_cpu)
-            tc = time.perf_counter()
-            o.step_batch(t, be - t + 1, ramp_velocity(cfg.u_lattice, be, cfg.ramp_steps))
-            tcpu += time.perf_counter() - tc
-            d = (be // args.diag) * args.diag
-            if t <= d <= be and d > cfg.ramp_steps:
-                cacc.add(oracle_forces(o, cfg, params, mesh, grids[-1].dx))
-            if n_cpu > batch and t <= d <= be:
-                log(f"cpu baseline: coarse step {be} of {n_cpu}, {tcpu:.1f}s")
-            t = be + 1
-        o.close()
-        c_true, _ = mlups(grids, n_cpu, tcpu)
-        cpu = {"value": round(c_true, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
-               "sample": f"{n_cpu} coarse step(s) of the same {sum(n_cells) / 1e6:.1f}M-cell, "
-                         f"{len(grids)}-level workload "
-                         f"({sum(c * 2 ** i for i, c in enumerate(n_cells)) * n_cpu / 1e6:.0f}M cell "
-                         f"updates), C oracle (FP32 mirror, OpenMP)", "seconds": round(tcpu, 2),
-               "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
-        if cacc.n_samples:
-            cc = averaged_coefficients(params, cacc.sums, cacc.n_samples)
-            cpu.update({"Cd_avg": float(f"{cc['Cd_avg']:.6g}"), "Cl_avg": float(f"{cc['Cl_avg']:.6g}"),
-                        "force_samples": cacc.n_samples})
-        log(f"cpu baseline {c_true:.1f} MLUPS on {threads} threads ({tcpu:.1f}s)")
+        cpu = cpu_baseline(cfg, grids, params, mesh, n_cpu, args.diag)
 
     serving = None
     if args.serving_requests > 0:

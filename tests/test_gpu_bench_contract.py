@@ -3,8 +3,10 @@ contract names, a roofline object whose fraction is its achieved rate over
 the HBM peak, and the CPU baseline of the oracle.  The GPU test runs the
 cube workload at a small resolution (a few million cells, 2 timed coarse
 steps) in a child process: FP32 after a short spin-up with force samples,
-and FP16 storage (its dtype names the 16-bit storage); the CPU test checks
-the refusal of N > 1."""
+and FP16 storage (its dtype names the 16-bit storage); the CPU tests check
+the refusal of N > 1 and the CPU baseline leg (bench.cpu_baseline) on a
+small case: its fields, its force cadence, and its averaged Cd/Cl against
+the FP64 oracle run with the main loop's cadence written out here."""
 from __future__ import annotations
 
 import json
@@ -23,6 +25,45 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "one GPU only" in r.stderr
     assert r.stdout.strip() == ""
+
+
+def test_cpu_baseline_leg_against_fp64_oracle():
+    sys.path.insert(0, str(ROOT))
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import bench
+    from oracle import Oracle, oracle_forces
+
+    from openludwig_amd import cases
+    from openludwig_amd.solver import build_case
+
+    cfg, mesh = cases.tiny_sphere(8, 2)
+    cfg = cfg.replace(ramp_steps=4)
+    grids, params, mesh = build_case(cfg, mesh)
+    n, diag = 24, 8
+    cpu = bench.cpu_baseline(cfg, grids, params, mesh, n, diag)
+    for key in ("value", "unit", "cores", "kind", "sample", "seconds", "cpu_model", "host_cpus"):
+        assert key in cpu
+    assert cpu["kind"] == "port" and cpu["unit"] == "MLUPS" and cpu["value"] > 0 and cpu["cores"] >= 1
+    # main.jl:214-243: batches of gpu_async_depth (8) at the inlet velocity
+    # of the batch end, a force sample after every batch holding a multiple
+    # of the cadence past the ramp: steps 8, 16, 24.  Every batch ends past
+    # the 4-step ramp, so every batch runs at the full inlet velocity.
+    assert cfg.gpu_async_depth == 8
+    assert cpu["force_samples"] == 3
+    o = Oracle(cfg, grids, params, "f64")
+    o.init_equilibrium()
+    cd, cl = [], []
+    for t in range(1, n + 1, 8):
+        o.step_batch(t, 8, float(cfg.u_lattice))
+        f = oracle_forces(o, cfg, params, mesh, grids[-1].dx)
+        cd.append(f["Cd"])
+        cl.append(f["Cl"])
+    o.close()
+    cd64, cl64 = sum(cd) / 3, sum(cl) / 3
+    assert abs(cpu["Cd_avg"] - cd64) <= 5e-3 * abs(cd64), (cpu["Cd_avg"], cd64)
+    assert abs(cpu["Cl_avg"] - cl64) <= 5e-3 * abs(cd64), (cpu["Cl_avg"], cl64)
+    one = bench.cpu_baseline(cfg, grids, params, mesh, 1, diag)
+    assert "force_samples" not in one and "1 coarse step(s)" in one["sample"]
 
 
 @pytest.mark.gpu
