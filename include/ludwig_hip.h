@@ -309,7 +309,7 @@ int lb_engine_check_finite(lb_engine* e, lb_ctx** diverged, int32_t cap, int32_t
  * of the stream-collide launches (LB_SCHED_*; default LB_SCHED_LINEAR, the
  * only one lb_step_batch uses) and the occupancy of the interior-block
  * stream-collide kernel, 5 or 6 waves per SIMD (the persistent form runs
- * at 5).                                                                 */
+ * at 5; default and lb_step_batch: 5 for FP32 storage, 6 for 16-bit).    */
 #define LB_SCHED_XCD 0        /* work-groups dealt to XCDs in contiguous runs  */
 #define LB_SCHED_LINEAR 1     /* work-group i takes half-tile i                */
 #define LB_SCHED_PERSISTENT 2 /* 5 work-groups per CU, each looping over the

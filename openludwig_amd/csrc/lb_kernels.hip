@@ -1038,9 +1038,17 @@ __device__ __forceinline__ void collide_tile(const LevelArgs* __restrict__ las, 
                     }
                 }
                 alpha = fmaxf(alpha, 0.0f);
+                // The equilibria are computed again rather than kept from
+                // the loop above: kept, they and their terms stay live across
+                // both loops and spill (the interior kernel spilled 63 VGPRs,
+                // 8 without them, and 6 waves per SIMD became the faster
+                // occupancy for 16-bit storage).  Opaque copies of the inputs
+                // stop the compiler from reusing the first loop's values.
+                float r2 = rho, ux2 = uxe, uy2 = uye, uz2 = uze;
+                asm volatile("" : "+v"(r2), "+v"(ux2), "+v"(uy2), "+v"(uz2));
 #pragma unroll
                 for (int k = 0; k < 27; k++) {
-                    float fe = feq_auto(rho, uxe, uye, uze, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k), a.comp);
+                    float fe = feq_auto(r2, ux2, uy2, uz2, WK(k), (float)CX(k), (float)CY(k), (float)CZ(k), a.comp);
                     A[k] = fe + alpha * (A[k] - fe);
                 }
             }

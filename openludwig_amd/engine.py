@@ -28,6 +28,12 @@ def ramp_table(cfg, n_steps: int) -> np.ndarray:
                      for i in range(n_batches)], dtype=np.float32)
 
 
+def default_waves(storage: int) -> int:
+    """The interior stream-collide kernel's default occupancy, as the library
+    picks it: 5 waves per SIMD for FP32 storage, 6 for 16-bit storage."""
+    return 5 if storage == N.LB_STORAGE_FP32 else 6
+
+
 class Engine:
     def __init__(self, storage: int = N.LB_STORAGE_FP32, strict_fp: bool = False, device: int = 0):
         self.lib = N.hip_lib()
@@ -37,7 +43,7 @@ class Engine:
             raise LudwigError(f"lb_engine_create failed ({rc})")
         self.storage = storage
         self.cases: dict = {}
-        self.options = {"graph": True, "schedule": N.LB_SCHED_LINEAR, "waves": 5}
+        self.options = {"graph": True, "schedule": N.LB_SCHED_LINEAR, "waves": default_waves(storage)}
 
     def close(self) -> None:
         if self._e:
@@ -115,7 +121,11 @@ class Engine:
         bad = {arr[i] for i in range(min(n.value, cap))}
         return [s for s in self.cases if s._ctx.value in bad]
 
-    def set_options(self, graph: bool = True, schedule: int = N.LB_SCHED_LINEAR, waves: int = 5) -> None:
+    def set_options(self, graph: bool = True, schedule: int = N.LB_SCHED_LINEAR, waves: int | None = None) -> None:
+        """Launch options (lb_engine_set_options); waves None: the storage's
+        default occupancy (5 waves per SIMD for FP32, 6 for 16-bit)."""
+        if waves is None:
+            waves = default_waves(self.storage)
         self._check(self.lib.lb_engine_set_options(self._e, int(graph), int(schedule), int(waves)),
                     "lb_engine_set_options")
         self.options = {"graph": bool(graph), "schedule": int(schedule), "waves": int(waves)}

@@ -27,13 +27,13 @@ def perturbed_state(grids, seed=1234, amp=0.01):
     return out
 
 
-def run_pair(cfg, grids, params, mesh, steps, strict, perturb=True, batch=4, storage=0, **solver_kw):
+def run_pair(cfg, grids, params, mesh, steps, strict, perturb=True, batch=4, storage=0, amp=0.01, **solver_kw):
     s = Solver(cfg, grids, params, mesh, strict_fp=strict, storage=storage, **solver_kw)
     o = Oracle(cfg, grids, params, "f32", storage=storage)
     s.init_equilibrium()
     o.init_equilibrium()
     if perturb:
-        for g, f in zip(grids, perturbed_state(grids)):
+        for g, f in zip(grids, perturbed_state(grids, amp=amp)):
             s.upload(g.level_id, "f", f)
             o.put(g.level_id, "f", f.astype(np.float64))
     t = 1

@@ -10,7 +10,10 @@
     2 w_k T_wall - g, wall cells driven to T_wall);
   * the symmetry plane at y-min (kernel_cumulant.jl:137, the mirror
     condition) with the force doubling of integrate_surface_forces!
-    (surface.jl:873-879).
+    (surface.jl:873-879);
+  * the positivity limiter engaged (kernel_cumulant.jl:564-599): a starting
+    state 60 % off equilibrium (f = w_k (1 + 0.6 U[-1,1])) at tau near 0.5
+    drives post-collision populations negative in many cells.
 
 Each runs the strict build and the FP32 oracle from the same perturbed
 state; bar: bit-identical populations, density, velocity (and g, T) on every
@@ -96,3 +99,18 @@ def test_symmetry_plane_and_force_doubling(hip_device):
     # doubling: the side force and the x/z moments vanish, Fx is twice the half-body sum
     assert fg["Fy"] == 0.0 and fg["Mx"] == 0.0 and fg["Mz"] == 0.0
     assert abs(fg["Fx"]) > 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", [0, 1])
+def test_positivity_limiter_engaged_strict_bitexact(hip_device, storage):
+    cfg, mesh = cases.tiny_sphere(8, 2, limiter="positivity")
+    grids, params, mesh = build_case(cfg, mesh)
+    s, o = run_pair(cfg, grids, params, mesh, steps=2, strict=True, amp=0.6, storage=storage)
+    _bitexact(compare(s, o, grids), f"positivity limiter engaged, storage {storage}")
+    # engaged: without the limiter the populations differ, and the oracle's
+    # unlimited run has negative ones (73 on level 1, 2 on level 2)
+    s0, o0 = run_pair(cfg.replace(limiter="none"), grids, params, mesh, steps=2, strict=True, amp=0.6,
+                      storage=storage)
+    assert np.count_nonzero(s.download(2, "f") != s0.download(2, "f")) > 0
+    assert (o0.get(1, "f") < 0).any() or (o0.get(2, "f") < 0).any()
