@@ -88,7 +88,10 @@ struct Level {
     int32_t ngh = 0;
     int64_t ngent = 0;
     int32_t* gent = nullptr;
-    int32_t* gpar = nullptr;  // per ghost entry: its 8 parent cells, (block << 9) | cell, or -1
+    int32_t* gpar = nullptr;     // FP32: per ghost entry, its 8 parent cells, (block << 9) | cell, or -1
+    int32_t* gstart = nullptr;   // 16-bit: per ghost block, its first entry (ngh + 1 values)
+    int32_t* gregion = nullptr;  // 16-bit: per ghost block, its 125 parent cells, (block << 9) | cell, or -1
+    uint32_t* gbmask = nullptr;  // 16-bit: per ghost block, the directions any of its entries serves
     int32_t pd[3] = {0, 0, 0};
     std::vector<int32_t> grid_host;  // host copy of grid, for the child level's ghost parents
     // Bouzidi link list
@@ -104,7 +107,7 @@ struct Level {
     // work lists over this level alone (LevelArgs index = level index):
     // blocks in processing order, ghost-entry chunks, Bouzidi-link chunks
     lbk::Work* wcol = nullptr;
-    int32_t n_gchunks = 0;
+    int32_t n_gchunks = 0;  // ghost fill work items: 256-entry chunks (FP32) or ghost blocks (16-bit)
     lbk::Work* wgh = nullptr;
     int32_t n_lchunks = 0;
     lbk::Work* wlk = nullptr;
@@ -546,6 +549,16 @@ void set_current(lb_ctx* c, int64_t t) {
     }
 }
 
+// Work list of items 0 .. n - 1 of one level (ghost blocks).
+int item_list(lb_ctx* c, int li, int32_t n, lbk::Work** out, int32_t* n_items) {
+    *n_items = n;
+    if (n == 0) return LB_OK;
+    std::vector<lbk::Work> w((size_t)n);
+    for (int32_t i = 0; i < n; i++) w[(size_t)i] = mkwork(li, i, 0, nullptr);
+    RC(dalloc(c, out, w.size()));
+    return h2d(c, *out, w.data(), w.size());
+}
+
 // Work list of 256-item chunks of one level (ghost entries, Bouzidi links).
 int chunk_list(lb_ctx* c, int li, int64_t n, lbk::Work** out, int32_t* n_chunks) {
     *n_chunks = (int32_t)((n + 255) / 256);
@@ -602,6 +615,9 @@ int upload_level_args(lb_ctx* c, int li) {
     }
     A.gent = L.gent;
     A.gpar = L.gpar;
+    A.gstart = L.gstart;
+    A.gregion = L.gregion;
+    A.gbmask = L.gbmask;
     A.ngent = L.ngent;
     A.links.n = L.nlinks;
     A.links.dst = L.ldst;
@@ -740,10 +756,14 @@ int engine_rebuild(lb_engine* e) {
                 (interior ? col : colb).push_back(w);
                 if (ddf) (interior ? tcol : tcolb).push_back(w);
             }
+            // ghost fill items: chunks of 256 entries (FP32), ghost blocks (16-bit)
+            const bool staged = e->storage != LB_STORAGE_FP32;
+            const int64_t n_items = staged ? (lev.ngent > 0 ? lev.ngh : 0) : (lev.ngent + 255) / 256;
             if (l > 1)
-                for (int64_t q = 0; q < lev.ngent; q += 256) {
-                    gh.push_back(mkwork(li, (int)q, ci, nullptr));
-                    if (ddf) tgh.push_back(mkwork(li, (int)q, ci, nullptr));
+                for (int64_t q = 0; q < n_items; q++) {
+                    const int item = (int)(staged ? q : 256 * q);
+                    gh.push_back(mkwork(li, item, ci, nullptr));
+                    if (ddf) tgh.push_back(mkwork(li, item, ci, nullptr));
                 }
             for (int64_t q = 0; q < lev.nlinks; q += 256) {
                 lk.push_back(mkwork(li, (int)q, ci, nullptr));
@@ -1121,14 +1141,17 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     }
     L.ngh = ngh;
     L.ngent = (int64_t)gent.size() / 2;
+    // The ghost fill's parent tables.  FP32 storage (ghost_kernel): per
+    // entry.  16-bit storage (ghost_kernel_staged): per ghost block.
+    const bool staged = c->cfg.storage != LB_STORAGE_FP32;
     // The 8 parent cells of every ghost entry, as the interpolation of
     // physics_interpolation.jl:4-127 addresses them: the fine cell centre at
     // (g - 0.5) / 2 in parent cells, its lower corner clamped to the domain,
     // each corner's parent block looked up in the parent grid (-1 where the
     // parent level holds no block).  Static, so the kernel reads them instead
     // of chasing the block origin and the parent grid per entry.
-    std::vector<int32_t> gpar((size_t)L.ngent * 8, -1);
-    if (L.ngent > 0) {
+    std::vector<int32_t> gpar(staged ? 0 : (size_t)L.ngent * 8, -1);
+    if (!staged && L.ngent > 0) {
         const Level& P = c->levels[(size_t)L.id - 2];
         const int pdx = P.pd[0], pdy = P.pd[1], pdz = P.pd[2];
         for (int64_t e = 0; e < L.ngent; e++) {
@@ -1151,6 +1174,44 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
                 if (pb >= (1 << 22)) return fail(c, LB_EINVAL, "level %d: parent block index %d too large", L.id, pb);
                 const int cc = ((pgx - 1) % 8) + 8 * ((pgy - 1) % 8) + 64 * ((pgz - 1) % 8);
                 gpar[(size_t)e * 8 + q] = (pb << 9) | cc;
+            }
+        }
+    }
+    // Per ghost block, for the staged interpolation of physics_interpolation.jl:
+    // 4-127: its entries' range, the union of their served directions, and
+    // its parent region.  A fine cell at local index l (0..7) of a block with
+    // origin B (a multiple of 8) has its centre at (g - 0.5) / 2 in parent
+    // cells, g = B + l + 1; the interpolation's lower corner is
+    // max(1, B/2 + l/2) and its upper corner B/2 + l/2 + 1, so every corner
+    // of the block lies in the 5^3 parent cells B/2 + r, r = 0..4 per axis.
+    // gregion holds (parent block << 9 | cell) for those 125 cells, looked up
+    // in the parent grid, or -1 where the parent level holds no block (or the
+    // cell is outside it).
+    const int32_t ngs = staged && L.ngent > 0 ? ngh : 0;
+    std::vector<int32_t> gstart((size_t)ngs + (staged ? 1 : 0), 0), gregion((size_t)ngs * 125, -1);
+    std::vector<uint32_t> gbmask((size_t)ngs, 0u);
+    for (int64_t e = 0; e < (ngs > 0 ? L.ngent : 0); e++) {
+        const int g = (gent[(size_t)e * 2] >> 9) - nb;
+        gstart[(size_t)g + 1]++;
+        gbmask[(size_t)g] |= (uint32_t)gent[(size_t)e * 2 + 1];
+    }
+    for (int32_t g = 0; g < ngs; g++) gstart[(size_t)g + 1] += gstart[(size_t)g];
+    if (ngs > 0) {
+        const Level& P = c->levels[(size_t)L.id - 2];
+        const int pdx = P.pd[0], pdy = P.pd[1], pdz = P.pd[2];
+        for (int32_t g = 0; g < ngs; g++) {
+            int base[3];
+            for (int ax = 0; ax < 3; ax++) base[ax] = borg[(size_t)(nb + g) * 3 + ax] / 2;
+            for (int r = 0; r < 125; r++) {
+                const int pgx = base[0] + r % 5, pgy = base[1] + (r / 5) % 5, pgz = base[2] + r / 25;
+                if (pgx < 1 || pgy < 1 || pgz < 1) continue;
+                const int pbx = (pgx - 1) / 8 + 1, pby = (pgy - 1) / 8 + 1, pbz = (pgz - 1) / 8 + 1;
+                if (pbx > pdx || pby > pdy || pbz > pdz) continue;
+                const int32_t pb = P.grid_host[(size_t)(pbx - 1) + (size_t)pdx * ((size_t)(pby - 1) + (size_t)pdy * (pbz - 1))];
+                if (pb < 0) continue;
+                if (pb >= (1 << 22)) return fail(c, LB_EINVAL, "level %d: parent block index %d too large", L.id, pb);
+                const int cc = ((pgx - 1) % 8) + 8 * ((pgy - 1) % 8) + 64 * ((pgz - 1) % 8);
+                gregion[(size_t)g * 125 + r] = (pb << 9) | cc;
             }
         }
     }
@@ -1181,6 +1242,11 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     RC(dalloc(c, &L.order, (size_t)nb));
     RC(dalloc(c, &L.gent, gent.size()));
     RC(dalloc(c, &L.gpar, gpar.size()));
+    if (!gstart.empty()) {
+        RC(dalloc(c, &L.gstart, gstart.size()));
+        RC(dalloc(c, &L.gregion, std::max<size_t>(gregion.size(), 1)));
+        RC(dalloc(c, &L.gbmask, std::max<size_t>(gbmask.size(), 1)));
+    }
     const size_t ng = (size_t)d->ptr_dim_x * d->ptr_dim_y * d->ptr_dim_z;
     RC(dalloc(c, &L.grid, ng));
     L.pd[0] = d->ptr_dim_x;
@@ -1188,6 +1254,9 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
     L.pd[2] = d->ptr_dim_z;
     if (!gent.empty()) RC(h2d(c, L.gent, gent.data(), gent.size()));
     if (!gpar.empty()) RC(h2d(c, L.gpar, gpar.data(), gpar.size()));
+    if (!gstart.empty()) RC(h2d(c, L.gstart, gstart.data(), gstart.size()));
+    if (!gregion.empty()) RC(h2d(c, L.gregion, gregion.data(), gregion.size()));
+    if (!gbmask.empty()) RC(h2d(c, L.gbmask, gbmask.data(), gbmask.size()));
     if (nb > 0) {
         RC(h2d(c, L.nbr, nbr.data(), nbr.size()));
         RC(h2d(c, L.borg, borg.data(), borg.size()));
@@ -1242,7 +1311,8 @@ int lb_level_upload(lb_ctx* c, const lb_level_desc* d) {
         L.grid_host = std::move(grid);
     }
     RC(build_links(c, L, d));
-    RC(chunk_list(c, L.id - 1, L.ngent, &L.wgh, &L.n_gchunks));
+    if (staged) RC(item_list(c, L.id - 1, L.ngent > 0 ? ngh : 0, &L.wgh, &L.n_gchunks));
+    else RC(chunk_list(c, L.id - 1, L.ngent, &L.wgh, &L.n_gchunks));
     RC(chunk_list(c, L.id - 1, L.nlinks, &L.wlk, &L.n_lchunks));
     RC(upload_level_args(c, L.id - 1));
     L.uploaded = true;

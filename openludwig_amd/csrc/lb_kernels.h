@@ -91,8 +91,16 @@ struct LevelArgs {
     float ptau, ptau_g;
     // ghost entries: [n][2] = (ghost block << 9 | cell, 27-bit direction mask)
     const int32_t* gent;
-    // per ghost entry: its 8 parent cells (parent block << 9 | cell, or -1)
+    // FP32 storage, per ghost entry: its 8 parent cells (parent block << 9 |
+    // cell, or -1)
     const int32_t* gpar;
+    // 16-bit storage (entries grouped by ghost block), per ghost block: its
+    // first entry ([ngh + 1]), its 125 parent cells (parent block << 9 |
+    // cell, or -1; cell r = rx + 5 ry + 25 rz of the parent region starting
+    // at half the block origin), the union of its entries' direction masks
+    const int32_t* gstart;
+    const int32_t* gregion;
+    const uint32_t* gbmask;
     int64_t ngent;
     BouzidiLinks links;
 };
