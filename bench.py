@@ -203,6 +203,7 @@ def main() -> int:
         cadence past the ramp (main.jl:226-243) and the divergence check."""
         t = t_first
         end = t_first + n - 1
+        last = time.perf_counter()
         while t <= end:
             be = min(t + batch - 1, end)
             solver.execute_timestep_batch(t, be - t + 1, ramp_velocity(cfg.u_lattice, be, cfg.ramp_steps))
@@ -210,6 +211,10 @@ def main() -> int:
             if sample and t <= d <= be and d > cfg.ramp_steps:
                 solver.check_finite()
                 acc.add(solver.compute_aerodynamics())
+            if sample and time.perf_counter() - last > 30.0:
+                solver.synchronize()
+                last = time.perf_counter()
+                log(f"spin-up: coarse step {be} of {end}")
             t = be + 1
 
     def sync() -> None:
