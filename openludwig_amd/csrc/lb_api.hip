@@ -96,9 +96,12 @@ struct Level {
     std::vector<int32_t> grid_host;  // host copy of grid, for the child level's ghost parents
     // Bouzidi link list
     int64_t nlinks = 0;
-    int64_t* ldst = nullptr;
-    int64_t* ls1 = nullptr;
-    int64_t* ls2 = nullptr;
+    // Bouzidi link slots: uint32 (lidx32, a population buffer below 2^32
+    // slots) or int64
+    void* ldst = nullptr;
+    void* ls1 = nullptr;
+    void* ls2 = nullptr;
+    bool lidx32 = false;
     float* la = nullptr;
     float* lb = nullptr;
     float* ltmp = nullptr;
@@ -430,15 +433,31 @@ int build_links(lb_ctx* c, Level& L, const lb_level_desc* d) {
         apply(ca);
         apply(cb);
     }
-    RC(dalloc(c, &L.ldst, n));
-    RC(dalloc(c, &L.ls1, n));
-    RC(dalloc(c, &L.ls2, n));
+    // 32-bit slot indices where the level's population buffer (ghost blocks
+    // and padding included) has fewer than 2^32 slots: 12 instead of 24 of
+    // the link's 40 bytes of data (gather and scatter read the link data on
+    // every substep).  wide_offsets keeps 64-bit indices too.
+    const uint64_t slots = ((uint64_t)nb + (uint64_t)L.ngh) * NC * Q + 512;
+    L.lidx32 = c->cfg.wide_offsets == 0 && slots < ((uint64_t)1 << 32);
+    auto upload_slots = [&](const std::vector<int64_t>& v, void** out) -> int {
+        if (L.lidx32) {
+            std::vector<uint32_t> v32(v.begin(), v.end());
+            uint32_t* p = nullptr;
+            RC(dalloc(c, &p, n));
+            *out = p;
+            return h2d(c, p, v32.data(), n);
+        }
+        int64_t* p = nullptr;
+        RC(dalloc(c, &p, n));
+        *out = p;
+        return h2d(c, p, v.data(), n);
+    };
+    RC(upload_slots(dst, &L.ldst));
+    RC(upload_slots(s1, &L.ls1));
+    RC(upload_slots(s2, &L.ls2));
     RC(dalloc(c, &L.la, n));
     RC(dalloc(c, &L.lb, n));
     RC(dalloc(c, &L.ltmp, n));
-    RC(h2d(c, L.ldst, dst.data(), n));
-    RC(h2d(c, L.ls1, s1.data(), n));
-    RC(h2d(c, L.ls2, s2.data(), n));
     RC(h2d(c, L.la, ca.data(), n));
     RC(h2d(c, L.lb, cb.data(), n));
     return LB_OK;
@@ -626,6 +645,7 @@ int upload_level_args(lb_ctx* c, int li) {
     A.links.a = L.la;
     A.links.b = L.lb;
     A.links.tmp = L.ltmp;
+    A.links.idx32 = L.lidx32 ? 1 : 0;
     return h2d(c, c->las_dev + li, &A, 1);
 }
 

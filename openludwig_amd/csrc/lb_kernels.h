@@ -47,15 +47,18 @@ struct CaseState {
 };
 
 // Bouzidi link list: corrected slot dst = a*f[s1] + b*f[s2], all indices
-// into one population buffer (block-major layout).
+// into one population buffer (block-major layout).  The slot indices are
+// uint32 when idx32 (a buffer below 2^32 slots), else int64.
 struct BouzidiLinks {
     int64_t n;
-    const int64_t* dst;
-    const int64_t* s1;
-    const int64_t* s2;
+    const void* dst;
+    const void* s1;
+    const void* s2;
     const float* a;
     const float* b;
     float* tmp;
+    int32_t idx32;
+    int32_t pad;
 };
 
 struct LevelArgs {

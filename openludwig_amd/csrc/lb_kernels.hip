@@ -1753,6 +1753,12 @@ __device__ __forceinline__ void* bouzidi_target(const LevelArgs& L, const CaseSt
     const int out = 1 - (int)(substep(L, S, j) & 1);
     return thermal ? (void*)L.g[out] : L.f[out];
 }
+// Slot index i of a link array (uint32 or int64 entries, uniform per level).
+__device__ __forceinline__ size_t link_slot(const BouzidiLinks& l, const void* p, int64_t i) {
+    typedef const __attribute__((address_space(1))) uint32_t GU32;
+    typedef const __attribute__((address_space(1))) int64_t GI64;
+    return l.idx32 ? (size_t)((GU32*)p)[i] : (size_t)((GI64*)p)[i];
+}
 template <int ST>
 __global__ void __launch_bounds__(256) bouzidi_gather(const LevelArgs* __restrict__ las,
                                                       const CaseState* __restrict__ states,
@@ -1765,8 +1771,8 @@ __global__ void __launch_bounds__(256) bouzidi_gather(const LevelArgs* __restric
     const int64_t i = (int64_t)wk.item + threadIdx.x;
     if (i >= l.n) return;
     const void* f = bouzidi_target(L, S, j, thermal);
-    const float w = wk_rt((int)((l.dst[i] >> 9) % 27));
-    const float f1 = Codec<ST>::load(f, (size_t)l.s1[i], w), f2 = Codec<ST>::load(f, (size_t)l.s2[i], w);
+    const float w = wk_rt((int)((link_slot(l, l.dst, i) >> 9) % 27));
+    const float f1 = Codec<ST>::load(f, link_slot(l, l.s1, i), w), f2 = Codec<ST>::load(f, link_slot(l, l.s2, i), w);
     l.tmp[i] = __fadd_rn(__fmul_rn(l.a[i], f1), __fmul_rn(l.b[i], f2));
 }
 template <int ST>
@@ -1781,8 +1787,9 @@ __global__ void __launch_bounds__(256) bouzidi_scatter(const LevelArgs* __restri
     const int64_t i = (int64_t)wk.item + threadIdx.x;
     if (i >= l.n) return;
     void* f = bouzidi_target(L, S, j, thermal);
-    const float w = wk_rt((int)((l.dst[i] >> 9) % 27));
-    Codec<ST>::store(f, (size_t)l.dst[i], l.tmp[i], w);
+    const size_t d = link_slot(l, l.dst, i);
+    const float w = wk_rt((int)((d >> 9) % 27));
+    Codec<ST>::store(f, d, l.tmp[i], w);
 }
 
 template <int ST>
