@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -232,6 +233,21 @@ int fail(lb_ctx* c, int code, const char* fmt, ...) {
         if (e_ != hipSuccess) return fail(ctx, LB_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
     } while (0)
 
+// Case buffers come from the device's default memory pool, allocated and
+// freed in the order of the case's stream: a finished case's buffers go back
+// to the pool without the device-wide wait and unmapping of hipFree (about
+// 16 ms for the ~100 buffers of a case), and later cases reuse them.  The
+// pool keeps up to 64 GiB of freed memory.
+void setup_pool(int dev) {
+    static std::atomic<bool> done[64];
+    if (dev < 0 || dev >= 64 || done[dev].exchange(true)) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t keep = 64ull << 30;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+}
+
 template <class T>
 int dalloc(lb_ctx* c, T** p, size_t n) {
     *p = nullptr;
@@ -240,7 +256,10 @@ int dalloc(lb_ctx* c, T** p, size_t n) {
         c->bytes += (int64_t)bytes;
         return LB_OK;
     }
-    hipError_t e = hipMalloc((void**)p, bytes);
+    setup_pool(c->device);
+    // usable from every stream once the case's stream has passed it
+    hipError_t e = hipMallocAsync((void**)p, bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
@@ -1006,8 +1025,11 @@ void lb_destroy(lb_ctx* c) {
                 (void)hipEventDestroy(e.first);
                 (void)hipEventDestroy(e.second);
             }
-    for (void* p : c->allocs) (void)hipFree(p);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream) {
+        for (void* p : c->allocs) (void)hipFreeAsync(p, c->stream);
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+    }
     delete c;
 }
 
